@@ -1,0 +1,196 @@
+/*
+ * mcml_detmath.h - deterministic arithmetic shared by the HIP transport kernels and any CPU
+ * code that must reproduce their photon trajectories bit for bit.
+ *
+ * Every function here is built only from IEEE-754 binary32 operations that are correctly
+ * rounded on both x86-64 (SSE/FMA) and gfx950 (VALU): +, -, *, /, sqrtf, fmaf, rintf,
+ * integer arithmetic and bit moves.  Compile every user with -ffp-contract=off and without
+ * fast-math, so that no a*b+c is silently fused on one side and not on the other.
+ *
+ * Contents
+ *   - multiply-with-carry uniform generator (Marsaglia 2003) with per-photon seeding by the
+ *     SplitMix64 finaliser (Steele, Lea & Flood, OOPSLA 2014);
+ *   - logarithm: exponent split + atanh series log(m) = 2*atanh((m-1)/(m+1))
+ *     (Abramowitz & Stegun 4.1.27), Cody-Waite split of ln 2;
+ *   - sin/cos of 2*pi*u: exact quarter-turn reduction + Taylor polynomials on |a| <= pi/4
+ *     (Cody & Waite, "Software Manual for the Elementary Functions", 1980);
+ *   - acos: asin Taylor series (A&S 4.4.40) on [0, 1/2] and acos(x) = 2 asin(sqrt((1-x)/2)).
+ * The series coefficients are the exact rational Taylor coefficients rounded to binary32.
+ * None of these reproduce any vendor math library; they are accurate to a few ulp and,
+ * more importantly, give identical bits on the host and on the GPU.
+ */
+#ifndef MCML_DETMATH_H
+#define MCML_DETMATH_H
+
+#include <stdint.h>
+#include <string.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define MCML_HD __host__ __device__ static inline
+#else
+#define MCML_HD static inline
+#endif
+
+/* ---------------------------------------------------------------- bit moves */
+MCML_HD uint32_t mcml_f2u(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
+MCML_HD float mcml_u2f(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
+
+/* ---------------------------------------------------------------- RNG */
+/* One photon's random stream: lag-1 multiply-with-carry, base 2^32.
+ * state = carry * 2^32 + x; next state = a * x + carry (fits in 64 bits for a < 2^32). */
+typedef struct {
+    uint64_t s;
+    uint32_t a;
+} mcml_mwc;
+
+MCML_HD uint32_t mcml_mwc_next(mcml_mwc* r)
+{
+    r->s = (uint64_t)r->a * (uint64_t)(uint32_t)r->s + (r->s >> 32);
+    return (uint32_t)r->s;
+}
+
+/* Uniform in (0, 1]: the top 24 bits plus one, times 2^-24.  Exact in binary32, never 0,
+ * so -log(u) is finite (the same half-open interval as the reference's generator). */
+MCML_HD float mcml_uniform(mcml_mwc* r)
+{
+    uint32_t x = mcml_mwc_next(r);
+    return (float)((x >> 8) + 1u) * 5.9604644775390625e-08f; /* 2^-24 */
+}
+
+MCML_HD uint64_t mcml_mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+#define MCML_GOLDEN 0x9E3779B97F4A7C15ull
+
+/* Stream of photon `k` of parameter set `set` under run seed `seed`.  The stream depends
+ * on (seed, set, k) only, so it does not matter which lane, thread or pass runs it.
+ * `table` holds MCML_MWC_TABLE_SIZE multipliers a in (2^31, 2^32) with a*2^32-1 a safe
+ * prime; the table size must be a power of two no larger than 4096. */
+MCML_HD mcml_mwc mcml_stream(uint64_t seed, uint32_t set, uint64_t k, const uint32_t* table,
+                             uint32_t table_size)
+{
+    mcml_mwc r;
+    uint64_t h = mcml_mix64(seed + MCML_GOLDEN * ((uint64_t)set + 1u));
+    h = mcml_mix64(h ^ (k * 0xD1B54A32D192ED03ull + MCML_GOLDEN));
+    uint32_t a = table[(uint32_t)(h >> 52) & (table_size - 1u)];
+    uint64_t h2 = mcml_mix64(h + MCML_GOLDEN);
+    uint32_t x = (uint32_t)h2;
+    uint32_t c = (uint32_t)(h2 >> 32);
+    if (c >= a) c -= a; /* a > 2^31, so one subtraction reduces c into [0, a) */
+    if (c == 0u && x == 0u) x = 1u;                   /* absorbing zero state */
+    if (c == a - 1u && x == 0xFFFFFFFFu) x = 0xFFFFFFFEu; /* the other fixed point */
+    r.s = ((uint64_t)c << 32) | x;
+    r.a = a;
+    return r;
+}
+
+/* ---------------------------------------------------------------- logarithm */
+#define MCML_LN2_HI 0.693359375f       /* 355/512, exact product with small integers */
+#define MCML_LN2_LO -2.12194440e-4f    /* ln 2 - LN2_HI */
+
+/* Natural logarithm of a positive finite float. */
+MCML_HD float mcml_log(float x)
+{
+    uint32_t ix = mcml_f2u(x);
+    int32_t e = 0;
+    if (ix < 0x00800000u) { /* subnormal: scale by 2^25 first */
+        x = x * 33554432.0f;
+        ix = mcml_f2u(x);
+        e = -25;
+    }
+    e += (int32_t)(ix >> 23) - 127;
+    uint32_t mb = (ix & 0x007FFFFFu) | 0x3F800000u;
+    if (mb > 0x3FB504F3u) { /* m > sqrt(2): halve it so m lies in (sqrt(1/2), sqrt(2)] */
+        mb -= 0x00800000u;
+        e += 1;
+    }
+    float m = mcml_u2f(mb);
+    float f = m - 1.0f;                /* exact */
+    float s = f / (2.0f + f);          /* |s| <= 0.1716 */
+    float z = s * s;
+    float p = fmaf(z, 1.818181818e-01f, 2.222222222e-01f); /* 2/11, 2/9 */
+    p = fmaf(z, p, 2.857142857e-01f);                      /* 2/7 */
+    p = fmaf(z, p, 4.000000000e-01f);                      /* 2/5 */
+    p = fmaf(z, p, 6.666666667e-01f);                      /* 2/3 */
+    float lm = fmaf(s * z, p, 2.0f * s);
+    float fe = (float)e;
+    return fmaf(fe, MCML_LN2_HI, fmaf(fe, MCML_LN2_LO, lm));
+}
+
+/* ---------------------------------------------------------------- sin / cos */
+#define MCML_TWO_PI 6.28318530717958647692f
+#define MCML_INV_TWO_PI 0.159154943091895335768f
+
+/* sin(2*pi*t) and cos(2*pi*t) for 0 <= t <= 1 (any |t| < 2^22 works). */
+MCML_HD void mcml_sincos_turns(float t, float* sn, float* cs)
+{
+    float q = rintf(t * 4.0f);          /* nearest quarter turn; t*4 is exact */
+    float f = t - q * 0.25f;            /* exact: |f| <= 1/8 */
+    int32_t k = ((int32_t)q) & 3;
+    float a = f * MCML_TWO_PI;          /* |a| <= pi/4 */
+    float z = a * a;
+    float ps = fmaf(z, 2.755731922e-06f, -1.984126984e-04f); /* 1/9!, -1/7! */
+    ps = fmaf(z, ps, 8.333333333e-03f);                       /* 1/5! */
+    ps = fmaf(z, ps, -1.666666667e-01f);                      /* -1/3! */
+    float s = fmaf(a * z, ps, a);
+    float pc = fmaf(z, -2.755731922e-07f, 2.480158730e-05f);  /* -1/10!, 1/8! */
+    pc = fmaf(z, pc, -1.388888889e-03f);                       /* -1/6! */
+    pc = fmaf(z, pc, 4.166666667e-02f);                        /* 1/4! */
+    pc = fmaf(z, pc, -0.5f);                                   /* -1/2! */
+    float c = fmaf(z, pc, 1.0f);
+    float rs, rc;
+    if (k == 0) { rs = s; rc = c; }
+    else if (k == 1) { rs = c; rc = -s; }
+    else if (k == 2) { rs = -s; rc = -c; }
+    else { rs = -c; rc = s; }
+    *sn = rs;
+    *cs = rc;
+}
+
+/* sin and cos of an angle in radians, |a| <= 4 (used for launch angles only). */
+MCML_HD void mcml_sincos(float a, float* sn, float* cs)
+{
+    mcml_sincos_turns(a * MCML_INV_TWO_PI, sn, cs);
+}
+
+/* ---------------------------------------------------------------- acos */
+/* asin(x) for |x| <= 1/2 from its Taylor series through x^23 (truncation < 2e-9 rel). */
+MCML_HD float mcml_asin_small(float x)
+{
+    float z = x * x;
+    float p = fmaf(z, 7.312525874e-03f, 8.390335810e-03f);
+    p = fmaf(z, p, 9.761609529e-03f);
+    p = fmaf(z, p, 1.155180090e-02f);
+    p = fmaf(z, p, 1.396484375e-02f);
+    p = fmaf(z, p, 1.735276442e-02f);
+    p = fmaf(z, p, 2.237215909e-02f);
+    p = fmaf(z, p, 3.038194444e-02f);
+    p = fmaf(z, p, 4.464285714e-02f);
+    p = fmaf(z, p, 7.500000000e-02f);
+    p = fmaf(z, p, 1.666666667e-01f);
+    return fmaf(x * z, p, x);
+}
+
+#define MCML_PIO2_HI 1.57079637050628662109f
+#define MCML_PIO2_LO -4.37113900018624283e-8f
+#define MCML_PI_HI 3.14159274101257324219f
+#define MCML_PI_LO -8.74227800037248566e-8f
+
+/* acos(t) for |t| <= 1; NaN outside [-1, 1] and for NaN input, like acosf. */
+MCML_HD float mcml_acos(float t)
+{
+    float a = fabsf(t);
+    if (!(a <= 1.0f)) return mcml_u2f(0x7FC00000u);
+    if (a <= 0.5f) return MCML_PIO2_HI - (mcml_asin_small(t) - MCML_PIO2_LO);
+    float w = sqrtf((1.0f - a) * 0.5f);
+    float r = 2.0f * mcml_asin_small(w); /* acos(|t|) */
+    if (t > 0.0f) return r;
+    return MCML_PI_HI - (r - MCML_PI_LO);
+}
+
+#endif /* MCML_DETMATH_H */

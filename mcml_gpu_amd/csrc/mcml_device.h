@@ -1,0 +1,93 @@
+/*
+ * mcml_device.h - device-resident descriptors shared by the HIP kernels and the host
+ * runtime that fills them.  Internal to libmcml; not part of the C ABI.
+ */
+#ifndef MCML_DEVICE_H
+#define MCML_DEVICE_H
+
+#include <stdint.h>
+#include "../../include/mcml.h"
+
+/* One layer of one parameter set, 32 B so a lane fetches it with two 16-B loads.  Index
+ * 0 is the upper ambient medium, 1..L tissue, L+1 the lower ambient medium. */
+struct DevLayer {
+    float z_min, z_max, mutr, mua, g, n, pad0, pad1;
+};
+
+/* One detector (index 1..S; entry 0 unused), 32 B. */
+struct DevDet {
+    float pos, radius, r2, cos_a, sin_a, cos_crit, pad0, pad1;
+};
+
+/* One tissue parameter set with its photon budget.  Photon k of set j is global queue
+ * ticket ticket_begin + k. */
+struct DevSet {
+    DevLayer layer[MCML_MAX_LAYERS + 2];
+    DevDet det[MCML_MAX_SDS + 1];
+    uint64_t ticket_begin;
+    uint64_t n_photons;
+    int32_t num_layers, num_sds;
+    float start_weight, det_reflectance;
+    int32_t src_oblique, det_oblique;
+    float src_radius, src_na, src_theta_max, src_sin_a, src_cos_a;
+    int32_t pad;
+};
+
+/* Work counters accumulated by the first-pass kernel (one atomic per lane at exit). */
+enum { CNT_PHOTONS = 0, CNT_STEPS = 1, CNT_BOUNDARY = 2, CNT_INTERACTION = 3, CNT_N = 4 };
+
+struct FirstPassArgs {
+    const DevSet* sets;
+    const uint64_t* set_begin; /* [nsets+1] ticket prefix: set j owns [set_begin[j], set_begin[j+1]) */
+    int32_t nsets;
+    int32_t chunk;             /* tickets a wavefront claims per queue access (>= 64) */
+    int32_t refill_min;        /* refill once this many lanes of the wavefront are idle */
+    int32_t pad;
+    uint64_t total;            /* total tickets */
+    uint64_t seed;
+    const uint32_t* mult;      /* MWC multiplier table */
+    unsigned long long* queue;     /* global ticket counter */
+    mcml_record* records;
+    unsigned long long* n_records; /* records appended (may exceed cap) */
+    uint64_t cap;
+    unsigned long long* counters;  /* [CNT_N] */
+};
+
+/* One replay job: photon k of set j, replayed for detector sds (1-based). */
+struct ReplayJob {
+    uint64_t k;
+    uint32_t set;
+    uint32_t sds;
+};
+
+struct ReplayArgs {
+    const DevSet* sets;
+    const ReplayJob* jobs;
+    uint64_t njobs;
+    uint64_t seed;
+    const uint32_t* mult;
+    unsigned long long* queue;
+    int32_t chunk;
+    int32_t num_layers;
+    /* outputs, structure of arrays over jobs */
+    float* weight;             /* [njobs] detected weight (0 if not re-detected) */
+    float* steps;              /* [njobs] step count as float */
+    float* pl;                 /* [num_layers * njobs], layer-major */
+    /* absorbance grids (NULL = off): per (set, sds) slab of nz*nr / nz cells */
+    unsigned long long* A_rz;
+    unsigned long long* A0_z;
+    int32_t num_sds, nr, nz, pad;
+    float dr, dz;
+};
+
+struct DumpArgs {
+    const float* weight;
+    const float* steps;
+    const float* pl;
+    uint64_t njobs;
+    int32_t num_layers;
+    int32_t pad;
+    float* out;                /* [njobs * (num_layers+2)], records as written to disk */
+};
+
+#endif /* MCML_DEVICE_H */

@@ -1,0 +1,550 @@
+/*
+ * transport.hip - gfx950 kernels of the photon-transport hot path.
+ *
+ *   mcml_first_pass  persistent hop-drop-spin kernel (reference: MCd + LaunchPhoton +
+ *                    Spin + Reflect + PhotonSurvive + detect, transport.cu:341-471,
+ *                    624-938).  Every lane owns one photon at a time; a wavefront claims
+ *                    photon tickets in chunks from one global counter and refills idle
+ *                    lanes in place, so lanes whose photon died or escaped are reused
+ *                    without leaving the kernel.  Tickets span all parameter sets of a
+ *                    batch, so one launch runs many sets with uneven budgets.
+ *   mcml_replay      re-runs detected photons from their (seed, set, photon) stream for
+ *                    one target detector, accumulating per-layer pathlength, step count
+ *                    and the absorbance grids (transport.cu:473-622, 941-1000).
+ *   mcml_dump        packs the replay outputs into the on-disk record layout
+ *                    [weight, PL_1..PL_L, steps] (io.cpp:302-310) with coalesced stores.
+ *
+ * Arithmetic: only correctly rounded IEEE binary32 operations, the deterministic
+ * functions of mcml_detmath.h and explicit operation order, compiled with
+ * -ffp-contract=off, so each photon's trajectory is a function of (seed, set, photon)
+ * alone and is bit-identical to a CPU evaluation of the same steps.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mcml_detmath.h"
+#include "mcml_device.h"
+#include "mwc_multipliers.h"
+
+#define RPI_F 0.318309886f /* 1/pi as the reference writes it (header.h:41) */
+
+namespace {
+
+/* ----------------------------------------------------------------- wave helpers */
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t rank_in(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t bcast_first(uint64_t v)
+{
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t lo = __shfl_xor((uint32_t)v, off, 64);
+        uint32_t hi = __shfl_xor((uint32_t)(v >> 32), off, 64);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+/* ----------------------------------------------------------------- photon state */
+struct Photon {
+    float x, y, z, dx, dy, dz, w;
+    int32_t layer;
+    int32_t steps;
+    int32_t first;
+    mcml_mwc rng;
+    /* the current layer's parameters, reloaded only when the layer changes */
+    float zmin, zmax, mutr, mua, g;
+};
+
+__device__ __forceinline__ void load_layer(Photon& p, const DevSet* S)
+{
+    const DevLayer L = S->layer[p.layer];
+    p.zmin = L.z_min;
+    p.zmax = L.z_max;
+    p.mutr = L.mutr;
+    p.mua = L.mua;
+    p.g = L.g;
+}
+
+/* Launch photon k of set `set` (LaunchPhoton, transport.cu:624-719). */
+__device__ __forceinline__ void launch(Photon& p, const DevSet* S, uint64_t seed, uint32_t set, uint64_t k,
+                                       const uint32_t* mult)
+{
+    p.rng = mcml_stream(seed, set, k, mult, MCML_MWC_TABLE_SIZE);
+    if (S->src_oblique) {
+        p.x = 0.0f; p.y = 0.0f; p.z = 0.0f;
+        p.dx = S->src_sin_a; p.dy = 0.0f; p.dz = S->src_cos_a;
+    } else {
+        if (S->src_radius == 0.0f) {
+            p.x = 0.0f; p.y = 0.0f; p.z = 0.0f;
+        } else {
+            float u_pos = mcml_uniform(&p.rng);
+            float u_az = mcml_uniform(&p.rng);
+            float rho = S->src_radius * u_pos;
+            float sa, ca;
+            mcml_sincos_turns(u_az, &sa, &ca);
+            p.x = rho * sa;
+            p.y = rho * ca;
+            p.z = 0.0f;
+        }
+        if (S->src_na == 0.0f) {
+            p.dx = 0.0f; p.dy = 0.0f; p.dz = 1.0f;
+        } else {
+            float u_dir = mcml_uniform(&p.rng);
+            float u_rot = mcml_uniform(&p.rng);
+            float th = S->src_theta_max * u_dir;
+            float st, ct, sr, cr;
+            mcml_sincos(th, &st, &ct);
+            mcml_sincos_turns(u_rot, &sr, &cr);
+            p.dz = ct;
+            p.dx = st * cr;
+            p.dy = st * sr;
+        }
+    }
+    p.layer = 1;
+    p.first = 1;
+    p.steps = 0;
+    p.w = S->start_weight;
+    load_layer(p, S);
+}
+
+/* Henyey-Greenstein deflection + azimuth + renormalisation (Spin, transport.cu:742-786). */
+__device__ __forceinline__ void spin(Photon& p)
+{
+    const float g = p.g;
+    float cost, temp;
+    if (g == 0.0f) {
+        cost = 2.0f * mcml_uniform(&p.rng) - 1.0f;
+    } else {
+        temp = (1.0f - g * g) / (1.0f - g + 2.0f * g * mcml_uniform(&p.rng));
+        cost = (1.0f + g * g - temp * temp) / (2.0f * g);
+    }
+    const float sint = sqrtf(1.0f - cost * cost);
+    float sinp, cosp;
+    mcml_sincos_turns(mcml_uniform(&p.rng), &sinp, &cosp);
+    temp = sqrtf(1.0f - p.dz * p.dz);
+    if (temp == 0.0f) {
+        p.dx = sint * cosp;
+        p.dy = sint * sinp;
+        p.dz = copysignf(cost, p.dz * cost);
+    } else {
+        const float cxp = (sint * (p.dx * p.dz * cosp - p.dy * sinp)) / temp + p.dx * cost;
+        const float cyp = (sint * (p.dy * p.dz * cosp + p.dx * sinp)) / temp + p.dy * cost;
+        p.dz = -sint * cosp * temp + p.dz * cost;
+        p.dx = cxp;
+        p.dy = cyp;
+    }
+    temp = 1.0f / sqrtf(p.dx * p.dx + p.dy * p.dy + p.dz * p.dz);
+    p.dx = p.dx * temp;
+    p.dy = p.dy * temp;
+    p.dz = p.dz * temp;
+}
+
+/* Fresnel boundary (Reflect, transport.cu:790-853): true = reflected. */
+__device__ __forceinline__ bool reflect(Photon& p, const DevSet* S, int new_layer)
+{
+    const float n1 = S->layer[p.layer].n;
+    const float n2 = S->layer[new_layer].n;
+    const float ci = fabsf(p.dz);
+    float r;
+    if (n1 == n2) {
+        p.layer = new_layer;
+        return false;
+    }
+    if (n1 > n2 && n2 * n2 < n1 * n1 * (1.0f - ci * ci)) {
+        p.dz = -p.dz;
+        return true;
+    }
+    if (ci == 1.0f) {
+        r = (n1 - n2) / (n1 + n2);
+        if (mcml_uniform(&p.rng) <= r * r) {
+            p.dz = -p.dz;
+            return true;
+        }
+        p.layer = new_layer;
+        return false;
+    }
+    float e = (n1 * n1) / (n2 * n2) * (1.0f - ci * ci);
+    r = 2.0f * sqrtf((1.0f - ci * ci) * (1.0f - e) * e * ci * ci);
+    e = e + (ci * ci) * (1.0f - 2.0f * e);
+    r = e * ((1.0f - e - r) / ((1.0f - e + r) * (e + r)));
+    if (mcml_uniform(&p.rng) <= r) {
+        p.dz = -p.dz;
+        return true;
+    }
+    r = n1 / n2;
+    e = r * r * (1.0f - ci * ci);
+    p.dx = p.dx * r;
+    p.dy = p.dy * r;
+    p.dz = copysignf(sqrtf(1.0f - e), p.dz);
+    p.layer = new_layer;
+    return false;
+}
+
+/* Does detector i accept the escaping photon (detect / detect_replay, transport.cu:872-1000)?
+ * Fiber mode: tilted elliptical footprint + NA cone; ring mode: annulus + NA cone. */
+__device__ __forceinline__ bool accepts(const Photon& p, const DevSet* S, int i)
+{
+    const DevDet D = S->det[i];
+    if (S->det_oblique) {
+        const float t = (p.x - D.pos) * D.cos_a;
+        if (t * t + p.y * p.y <= D.r2) {
+            const float uz = p.dx * D.sin_a + fabsf(p.dz * D.cos_a);
+            return fabsf(uz) >= D.cos_crit;
+        }
+        return false;
+    }
+    if (fabsf(p.dz) >= D.cos_crit) {
+        const float dist = sqrtf(p.x * p.x + p.y * p.y);
+        return dist >= D.pos - D.radius && dist <= D.pos + D.radius;
+    }
+    return false;
+}
+
+/* Weight credited to detector i (ring mode applies the arc fraction acos(t)/pi). */
+__device__ __forceinline__ float credited(const Photon& p, const DevSet* S, int i)
+{
+    if (S->det_oblique) return p.w;
+    const DevDet D = S->det[i];
+    const float dist = sqrtf(p.x * p.x + p.y * p.y);
+    float t = (dist * dist + D.pos * D.pos - D.radius * D.radius) / (2.0f * dist * D.pos);
+    if (t > 1.0f) t = 1.0f;
+    return p.w * mcml_acos(t) * RPI_F;
+}
+
+enum StepOutcome { ST_ALIVE = 0, ST_DEAD = 1, ST_DETECTED = 2 };
+
+/* Hop, boundary, drop, spin for one step; returns the outcome.  `target` = 0 tests every
+ * detector (first pass), otherwise only that detector (replay).  On ST_DETECTED, *mask
+ * holds the accepting detectors (bit i = SDS i). */
+template <bool REPLAY>
+__device__ __forceinline__ int hop_drop_spin(Photon& p, const DevSet* S, int target, uint32_t* mask, float* pl_lane,
+                                             int pl_stride, unsigned long long* a_rz, unsigned long long* a0_z,
+                                             const ReplayArgs* ra, uint64_t& rl_idx, uint64_t& rl_w,
+                                             uint32_t& n_bnd, uint32_t& n_int)
+{
+    float s = -mcml_log(mcml_uniform(&p.rng)) * p.mutr;
+    int new_layer = p.layer;
+    if (p.z + s * p.dz < p.zmin) {
+        new_layer--;
+        s = (p.zmin - p.z) / p.dz;
+    }
+    if (p.z + s * p.dz > p.zmax) {
+        new_layer++;
+        s = (p.zmax - p.z) / p.dz;
+    }
+    p.x += p.dx * s;
+    p.y += p.dy * s;
+    p.z += p.dz * s;
+    p.steps++;
+    if (REPLAY) pl_lane[(p.layer - 1) * pl_stride] += s;
+    if (p.z > p.zmax) p.z = p.zmax;
+    if (p.z < p.zmin) p.z = p.zmin;
+
+    if (new_layer != p.layer) {
+        n_bnd++;
+        const int temp_layer = p.layer;
+        if (!reflect(p, S, new_layer)) {
+            if (new_layer == 0) {
+                uint32_t m = 0;
+                if (REPLAY) {
+                    if (accepts(p, S, target)) m = 1u << target;
+                } else {
+                    for (int i = 1; i <= S->num_sds; i++)
+                        if (accepts(p, S, i)) m |= 1u << i;
+                }
+                if (m) {
+                    *mask = m;
+                    return ST_DETECTED;
+                }
+                if (mcml_uniform(&p.rng) > S->det_reflectance) return ST_DEAD;
+                p.dz = -p.dz; /* the detector reflects the photon back, direction kept */
+                p.z = -p.z;
+                p.layer = temp_layer;
+            } else if (new_layer > S->num_layers) {
+                return ST_DEAD;
+            } else {
+                load_layer(p, S);
+            }
+        }
+        /* a boundary step ends with s = 0: no drop, no spin */
+    } else if (s > 0.0f) {
+        n_int++;
+        const float w_temp = rintf(p.mua * p.mutr * p.w);
+        p.w -= w_temp;
+        if (REPLAY && a_rz != nullptr) {
+            float qz = p.z / ra->dz;
+            qz = fminf(fmaxf(qz, 0.0f), (float)(ra->nz - 1));
+            const int zb = (int)qz;
+            if (p.first) {
+                atomicAdd(&a0_z[zb], (unsigned long long)w_temp);
+            } else {
+                float qr = sqrtf(p.x * p.x + p.y * p.y) / ra->dr;
+                qr = fminf(fmaxf(qr, 0.0f), (float)(ra->nr - 1));
+                const uint64_t idx = (uint64_t)zb * (uint64_t)ra->nr + (uint64_t)(int)qr;
+                if (idx == rl_idx) {
+                    rl_w += (uint64_t)w_temp;
+                } else {
+                    if (rl_w != 0) atomicAdd(&a_rz[rl_idx], (unsigned long long)rl_w);
+                    rl_idx = idx;
+                    rl_w = (uint64_t)w_temp;
+                }
+            }
+        }
+        p.first = 0;
+        spin(p);
+    }
+    /* PhotonSurvive (transport.cu:855-870).  The reference applies the step cap in the
+     * first pass only; a replayed photon was detected within the cap, so applying it in
+     * replay as well changes nothing for detected photons and bounds every replay loop. */
+    if (p.steps >= MCML_MAX_SCATTER) return ST_DEAD;
+    if (p.w > (float)MCML_WEIGHTI) return ST_ALIVE;
+    if (p.w == 0.0f) return ST_DEAD;
+    if (mcml_uniform(&p.rng) < MCML_CHANCE) {
+        p.w = rintf(p.w / MCML_CHANCE);
+        return ST_ALIVE;
+    }
+    return ST_DEAD;
+}
+
+__device__ __forceinline__ uint32_t set_of_ticket(const uint64_t* set_begin, int nsets, uint64_t t)
+{
+    int lo = 0, hi = nsets;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (set_begin[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return (uint32_t)lo;
+}
+
+} // namespace
+
+/* ----------------------------------------------------------------- first pass */
+extern "C" __global__ void __launch_bounds__(256) mcml_first_pass(FirstPassArgs a)
+{
+    Photon p;
+    bool alive = false;
+    const DevSet* S = a.sets;
+    uint32_t set = 0;
+    uint64_t key = 0;
+    /* wavefront-local ticket range [q_next, q_end); uniform across the wavefront */
+    uint64_t q_next = 0, q_end = 0;
+    bool q_dry = false;
+    uint64_t c_photons = 0, c_steps = 0;
+    uint32_t c_bnd = 0, c_int = 0;
+    uint64_t c_bnd64 = 0, c_int64 = 0;
+    uint64_t rl_idx = ~0ull, rl_w = 0;
+    const int refill_min = a.refill_min < 1 ? 1 : a.refill_min;
+
+    for (;;) {
+        const uint64_t dead = __ballot(!alive);
+        const int ndead = __popcll(dead);
+        if (!q_dry && (ndead >= refill_min || ndead == 64)) {
+            const uint32_t rank = rank_in(dead);
+            const uint64_t avail = q_end - q_next;
+            uint64_t t = ~0ull;
+            if (!alive && rank < avail) t = q_next + rank;
+            if ((uint64_t)ndead > avail) {
+                unsigned long long base = 0;
+                if (lane_id() == (uint32_t)(__ffsll((long long)__ballot(1)) - 1))
+                    base = atomicAdd(a.queue, (unsigned long long)a.chunk);
+                base = bcast_first(base);
+                uint64_t cend = base + (uint64_t)a.chunk;
+                if (cend > a.total) cend = a.total;
+                if (base >= a.total) {
+                    q_dry = true;
+                    cend = base;
+                }
+                if (!alive && rank >= avail) {
+                    const uint64_t tt = base + (rank - avail);
+                    if (tt < cend) t = tt;
+                }
+                const uint64_t used = base + ((uint64_t)ndead - avail);
+                q_next = used < cend ? used : cend;
+                q_end = cend;
+            } else {
+                q_next += (uint64_t)ndead;
+            }
+            if (t != ~0ull) {
+                set = a.nsets > 1 ? set_of_ticket(a.set_begin, a.nsets, t) : 0u;
+                S = a.sets + set;
+                const uint64_t k = t - a.set_begin[set];
+                key = ((uint64_t)set << 40) | k;
+                launch(p, S, a.seed, set, k, a.mult);
+                alive = true;
+                c_photons++;
+            }
+        }
+        if (__ballot(alive) == 0) break;
+        if (alive) {
+            uint32_t mask = 0;
+            const int st = hop_drop_spin<false>(p, S, 0, &mask, nullptr, 0, nullptr, nullptr, nullptr, rl_idx, rl_w,
+                                                c_bnd, c_int);
+            if (st != ST_ALIVE) {
+                alive = false;
+                c_steps += (uint64_t)p.steps;
+                c_bnd64 += c_bnd;
+                c_int64 += c_int;
+                c_bnd = 0;
+                c_int = 0;
+                if (st == ST_DETECTED) {
+                    const unsigned long long base = atomicAdd(a.n_records, (unsigned long long)__popc(mask));
+                    unsigned long long idx = base;
+                    for (int i = 1; i <= S->num_sds; i++) {
+                        if (!(mask & (1u << i))) continue;
+                        if (idx < a.cap) {
+                            mcml_record r;
+                            r.key = key;
+                            r.sds = (uint32_t)i;
+                            r.weight = credited(p, S, i);
+                            a.records[idx] = r;
+                        }
+                        idx++;
+                    }
+                }
+            }
+        }
+    }
+    const uint64_t s0 = wave_sum(c_photons), s1 = wave_sum(c_steps), s2 = wave_sum(c_bnd64), s3 = wave_sum(c_int64);
+    if (lane_id() == 0) {
+        atomicAdd(&a.counters[CNT_PHOTONS], (unsigned long long)s0);
+        atomicAdd(&a.counters[CNT_STEPS], (unsigned long long)s1);
+        atomicAdd(&a.counters[CNT_BOUNDARY], (unsigned long long)s2);
+        atomicAdd(&a.counters[CNT_INTERACTION], (unsigned long long)s3);
+    }
+}
+
+/* ----------------------------------------------------------------- replay */
+#define REPLAY_BLOCK 256
+
+extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArgs a)
+{
+    /* per-lane pathlength accumulators, layer-major so lanes never share a bank word */
+    __shared__ float pl_lds[MCML_MAX_LAYERS * REPLAY_BLOCK];
+    const int tid = threadIdx.x;
+    float* pl_lane = pl_lds + tid;
+    const int L = a.num_layers;
+    uint64_t q_next = 0, q_end = 0;
+    bool q_dry = false;
+    uint32_t dummy_b = 0, dummy_i = 0;
+
+    for (;;) {
+        /* every lane takes a fresh job each round: jobs are independent and short */
+        const uint64_t all = __ballot(1);
+        const int nl = __popcll(all);
+        const uint32_t rank = rank_in(all);
+        uint64_t job = ~0ull;
+        if (!q_dry) {
+            const uint64_t avail = q_end - q_next;
+            if (rank < avail) job = q_next + rank;
+            if ((uint64_t)nl > avail) {
+                unsigned long long base = 0;
+                if (rank == 0) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
+                base = bcast_first(base);
+                uint64_t cend = base + (uint64_t)a.chunk;
+                if (cend > a.njobs) cend = a.njobs;
+                if (base >= a.njobs) {
+                    q_dry = true;
+                    cend = base;
+                }
+                if (rank >= avail) {
+                    const uint64_t jj = base + (rank - avail);
+                    if (jj < cend) job = jj;
+                }
+                const uint64_t used = base + ((uint64_t)nl - avail);
+                q_next = used < cend ? used : cend;
+                q_end = cend;
+            } else {
+                q_next += (uint64_t)nl;
+            }
+        }
+        if (__ballot(job != ~0ull) == 0) break;
+        if (job != ~0ull) {
+            const ReplayJob J = a.jobs[job];
+            const DevSet* S = a.sets + J.set;
+            Photon p;
+            launch(p, S, a.seed, J.set, J.k, a.mult);
+            for (int l = 0; l < L; l++) pl_lane[l * REPLAY_BLOCK] = 0.0f;
+            unsigned long long* a_rz = nullptr;
+            unsigned long long* a0_z = nullptr;
+            if (a.A_rz != nullptr) {
+                const uint64_t slab = (uint64_t)J.set * (uint64_t)a.num_sds + (uint64_t)(J.sds - 1);
+                a_rz = a.A_rz + slab * (uint64_t)a.nz * (uint64_t)a.nr;
+                a0_z = a.A0_z + slab * (uint64_t)a.nz;
+            }
+            uint64_t rl_idx = ~0ull, rl_w = 0;
+            uint32_t mask = 0;
+            int st;
+            do {
+                st = hop_drop_spin<true>(p, S, (int)J.sds, &mask, pl_lane, REPLAY_BLOCK, a_rz, a0_z, &a, rl_idx,
+                                         rl_w, dummy_b, dummy_i);
+            } while (st == ST_ALIVE);
+            if (a_rz != nullptr && rl_w != 0) atomicAdd(&a_rz[rl_idx], (unsigned long long)rl_w);
+            a.weight[job] = (st == ST_DETECTED) ? credited(p, S, (int)J.sds) : 0.0f;
+            a.steps[job] = (float)p.steps;
+            for (int l = 0; l < L; l++) a.pl[(uint64_t)l * a.njobs + job] = pl_lane[l * REPLAY_BLOCK];
+        }
+    }
+}
+
+/* ----------------------------------------------------------------- dump */
+/* Thread t writes float t of the record image: record t / (L+2), field t % (L+2). */
+extern "C" __global__ void __launch_bounds__(256) mcml_dump(DumpArgs a)
+{
+    const uint64_t stride = (uint64_t)a.num_layers + 2u;
+    const uint64_t total = a.njobs * stride;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t rec = t / stride;
+        const uint32_t f = (uint32_t)(t - rec * stride);
+        float v;
+        if (f == 0) v = a.weight[rec];
+        else if (f <= (uint32_t)a.num_layers) v = a.pl[(uint64_t)(f - 1) * a.njobs + rec];
+        else v = a.steps[rec];
+        a.out[t] = v;
+    }
+}
+
+/* ----------------------------------------------------------------- host launchers */
+#include "mcml_launch.h"
+
+hipError_t mcml_launch_first_pass(const FirstPassArgs& a, int grid, int block, hipStream_t st)
+{
+    hipLaunchKernelGGL(mcml_first_pass, dim3(grid), dim3(block), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t mcml_launch_replay(const ReplayArgs& a, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(mcml_replay, dim3(grid), dim3(REPLAY_BLOCK), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t mcml_launch_dump(const DumpArgs& a, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(mcml_dump, dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t mcml_occupancy_first_pass(int block, int* blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_first_pass, block, 0);
+}
+
+hipError_t mcml_occupancy_replay(int* blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_replay, REPLAY_BLOCK, 0);
+}
+
+int mcml_replay_block() { return REPLAY_BLOCK; }
