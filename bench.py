@@ -44,7 +44,20 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--bpc", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=float, default=0,
+                    help="only run a plain device copy of this many bytes (HBM counter scale)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        nbytes = int(args.calibrate_copy)
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "read_bytes": nbytes, "write_bytes": nbytes}))
+        return
 
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "

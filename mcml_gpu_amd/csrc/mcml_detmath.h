@@ -10,12 +10,13 @@
  * Contents
  *   - multiply-with-carry uniform generator (Marsaglia 2003) with per-photon seeding by the
  *     SplitMix64 finaliser (Steele, Lea & Flood, OOPSLA 2014);
- *   - logarithm: exponent split + atanh series log(m) = 2*atanh((m-1)/(m+1))
- *     (Abramowitz & Stegun 4.1.27), Cody-Waite split of ln 2;
+ *   - logarithm: exponent split, a fitted polynomial for log(1+f) on
+ *     [sqrt(1/2)-1, sqrt(2)-1] (fit_log1p.py), Cody-Waite split of ln 2;
  *   - sin/cos of 2*pi*u: exact quarter-turn reduction + Taylor polynomials on |a| <= pi/4
  *     (Cody & Waite, "Software Manual for the Elementary Functions", 1980);
  *   - acos: asin Taylor series (A&S 4.4.40) on [0, 1/2] and acos(x) = 2 asin(sqrt((1-x)/2)).
- * The series coefficients are the exact rational Taylor coefficients rounded to binary32.
+ * The sin/cos/asin coefficients are the exact rational Taylor coefficients rounded to
+ * binary32; the log coefficients come from fit_log1p.py.
  * None of these reproduce any vendor math library; they are accurate to a few ulp and,
  * more importantly, give identical bits on the host and on the GPU.
  */
@@ -93,7 +94,9 @@ MCML_HD mcml_mwc mcml_stream(uint64_t seed, uint32_t set, uint64_t k, const uint
 #define MCML_LN2_HI 0.693359375f       /* 355/512, exact product with small integers */
 #define MCML_LN2_LO -2.12194440e-4f    /* ln 2 - LN2_HI */
 
-/* Natural logarithm of a positive finite float. */
+/* Natural logarithm of a positive finite float: x = 2^e * m with m in (sqrt(1/2), sqrt(2)],
+ * log(m) = f + f^2 P(f), f = m - 1, P of degree 8 fitted for minimal maximum relative error
+ * on that interval (fit_log1p.py; 4e-9 before rounding the coefficients to binary32). */
 MCML_HD float mcml_log(float x)
 {
     uint32_t ix = mcml_f2u(x);
@@ -109,15 +112,16 @@ MCML_HD float mcml_log(float x)
         mb -= 0x00800000u;
         e += 1;
     }
-    float m = mcml_u2f(mb);
-    float f = m - 1.0f;                /* exact */
-    float s = f / (2.0f + f);          /* |s| <= 0.1716 */
-    float z = s * s;
-    float p = fmaf(z, 1.818181818e-01f, 2.222222222e-01f); /* 2/11, 2/9 */
-    p = fmaf(z, p, 2.857142857e-01f);                      /* 2/7 */
-    p = fmaf(z, p, 4.000000000e-01f);                      /* 2/5 */
-    p = fmaf(z, p, 6.666666667e-01f);                      /* 2/3 */
-    float lm = fmaf(s * z, p, 2.0f * s);
+    float f = mcml_u2f(mb) - 1.0f; /* exact */
+    float p = fmaf(f, -7.623060793e-02f, 1.290550232e-01f);
+    p = fmaf(f, p, -1.324298084e-01f);
+    p = fmaf(f, p, 1.418163478e-01f);
+    p = fmaf(f, p, -1.660905629e-01f);
+    p = fmaf(f, p, 2.000198513e-01f);
+    p = fmaf(f, p, -2.500158548e-01f);
+    p = fmaf(f, p, 3.333332539e-01f);
+    p = fmaf(f, p, -4.999998808e-01f);
+    float lm = fmaf(f * f, p, f);
     float fe = (float)e;
     return fmaf(fe, MCML_LN2_HI, fmaf(fe, MCML_LN2_LO, lm));
 }

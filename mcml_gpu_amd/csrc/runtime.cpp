@@ -253,7 +253,7 @@ extern "C" mcml_ctx* mcml_create(int device, int* err)
         return nullptr;
     }
     c->cu_count = prop.multiProcessorCount;
-    snprintf(c->name, sizeof(c->name), "%s", prop.name);
+    snprintf(c->name, sizeof(c->name), "%s", prop.name[0] ? prop.name : prop.gcnArchName);
     if (hipMalloc((void**)&c->d_mult, sizeof(uint32_t) * MCML_MWC_TABLE_SIZE) != hipSuccess ||
         hipMemcpy(c->d_mult, mcml_mwc_multipliers, sizeof(uint32_t) * MCML_MWC_TABLE_SIZE,
                   hipMemcpyHostToDevice) != hipSuccess) {

@@ -62,8 +62,9 @@ struct Photon {
     int32_t steps;
     int32_t first;
     mcml_mwc rng;
-    /* the current layer's parameters, reloaded only when the layer changes */
-    float zmin, zmax, mutr, mua, g;
+    /* the current layer's parameters, reloaded only when the layer changes; the
+     * Henyey-Greenstein constants are the same float expressions the step would form */
+    float zmin, zmax, mutr, albedo, g, hg_a, hg_b, hg_c, hg_d, hg_inv2g;
 };
 
 __device__ __forceinline__ void load_layer(Photon& p, const DevSet* S)
@@ -72,8 +73,13 @@ __device__ __forceinline__ void load_layer(Photon& p, const DevSet* S)
     p.zmin = L.z_min;
     p.zmax = L.z_max;
     p.mutr = L.mutr;
-    p.mua = L.mua;
+    p.albedo = L.mua * L.mutr;
     p.g = L.g;
+    p.hg_a = 1.0f - L.g * L.g;
+    p.hg_b = 1.0f - L.g;
+    p.hg_c = 2.0f * L.g;
+    p.hg_d = 1.0f + L.g * L.g;
+    p.hg_inv2g = 1.0f / (2.0f * L.g);
 }
 
 /* Launch photon k of set `set` (LaunchPhoton, transport.cu:624-719). */
@@ -118,36 +124,42 @@ __device__ __forceinline__ void launch(Photon& p, const DevSet* S, uint64_t seed
     load_layer(p, S);
 }
 
-/* Henyey-Greenstein deflection + azimuth + renormalisation (Spin, transport.cu:742-786). */
+/* Henyey-Greenstein deflection + azimuth + renormalisation (Spin, transport.cu:742-786),
+ * written with one division and one square root on the common path:
+ * P = sin(theta) sqrt(1-dz^2), Q = sin(theta) / sqrt(1-dz^2), and a second-order
+ * rsqrt about 1 for the renormalisation (the norm never drifts more than a few ulp). */
 __device__ __forceinline__ void spin(Photon& p)
 {
-    const float g = p.g;
-    float cost, temp;
-    if (g == 0.0f) {
+    float cost;
+    if (p.g == 0.0f) {
         cost = 2.0f * mcml_uniform(&p.rng) - 1.0f;
     } else {
-        temp = (1.0f - g * g) / (1.0f - g + 2.0f * g * mcml_uniform(&p.rng));
-        cost = (1.0f + g * g - temp * temp) / (2.0f * g);
+        const float temp = p.hg_a / (p.hg_b + p.hg_c * mcml_uniform(&p.rng));
+        cost = (p.hg_d - temp * temp) * p.hg_inv2g;
     }
-    const float sint = sqrtf(1.0f - cost * cost);
+    const float a = 1.0f - cost * cost;
     float sinp, cosp;
     mcml_sincos_turns(mcml_uniform(&p.rng), &sinp, &cosp);
-    temp = sqrtf(1.0f - p.dz * p.dz);
-    if (temp == 0.0f) {
+    const float b = 1.0f - p.dz * p.dz;
+    if (b == 0.0f) {
+        const float sint = sqrtf(a);
         p.dx = sint * cosp;
         p.dy = sint * sinp;
         p.dz = copysignf(cost, p.dz * cost);
     } else {
-        const float cxp = (sint * (p.dx * p.dz * cosp - p.dy * sinp)) / temp + p.dx * cost;
-        const float cyp = (sint * (p.dy * p.dz * cosp + p.dx * sinp)) / temp + p.dy * cost;
-        p.dz = -sint * cosp * temp + p.dz * cost;
+        const float P = sqrtf(a * b);
+        const float Q = P / b;
+        const float cxp = Q * (p.dx * p.dz * cosp - p.dy * sinp) + p.dx * cost;
+        const float cyp = Q * (p.dy * p.dz * cosp + p.dx * sinp) + p.dy * cost;
+        p.dz = -P * cosp + p.dz * cost;
         p.dx = cxp;
         p.dy = cyp;
     }
-    temp = 1.0f / sqrtf(p.dx * p.dx + p.dy * p.dy + p.dz * p.dz);
-    p.dx = p.dx * temp;
-    p.dy = p.dy * temp;
-    p.dz = p.dz * temp;
+    const float d = p.dx * p.dx + p.dy * p.dy + p.dz * p.dz - 1.0f;
+    const float r = fmaf(d, fmaf(d, 0.375f, -0.5f), 1.0f);
+    p.dx = p.dx * r;
+    p.dy = p.dy * r;
+    p.dz = p.dz * r;
 }
 
 /* Fresnel boundary (Reflect, transport.cu:790-853): true = reflected. */
@@ -280,7 +292,7 @@ __device__ __forceinline__ int hop_drop_spin(Photon& p, const DevSet* S, int tar
         /* a boundary step ends with s = 0: no drop, no spin */
     } else if (s > 0.0f) {
         n_int++;
-        const float w_temp = rintf(p.mua * p.mutr * p.w);
+        const float w_temp = rintf(p.albedo * p.w);
         p.w -= w_temp;
         if (REPLAY && a_rz != nullptr) {
             float qz = p.z / ra->dz;

@@ -151,34 +151,46 @@ void oracle_launch(const mcml_derived* d, uint64_t seed, uint32_t set, uint64_t 
 
 /* Spin, transport.cu:742-786: Henyey-Greenstein (or isotropic for g == 0) deflection,
  * uniform azimuth, MCML direction update with the |dz| == 1 special case, renormalise.
+ * Same mathematics, fewer divisions and square roots than the reference's expression:
+ *   cos(theta) = (1 + g^2 - t^2) * (1 / 2g)              (reference divides by 2g)
+ *   P = sqrt((1 - cos^2)(1 - dz^2)), Q = P / (1 - dz^2)    (= sin(theta) * sqrt(1-dz^2) and
+ *                                                           sin(theta) / sqrt(1-dz^2))
+ *   renormalisation by 1 - d/2 + 3d^2/8, d = |u|^2 - 1     (second-order rsqrt about 1;
+ *                                                           |d| stays at a few ulp)
  * Returns cos(theta) for the tests. */
 float oracle_spin(oracle_photon* p, float g)
 {
-    float cost, sint, cosp, sinp, temp;
+    float cost;
     if (g == 0.0f) {
         cost = 2.0f * urand(p) - 1.0f;
     } else {
-        temp = (1.0f - g * g) / (1.0f - g + 2.0f * g * urand(p));
-        cost = (1.0f + g * g - temp * temp) / (2.0f * g);
+        float inv2g = 1.0f / (2.0f * g);
+        float temp = (1.0f - g * g) / (1.0f - g + 2.0f * g * urand(p));
+        cost = (1.0f + g * g - temp * temp) * inv2g;
     }
-    sint = sqrtf(1.0f - cost * cost);
+    float a = 1.0f - cost * cost;
+    float sinp, cosp;
     mcml_sincos_turns(urand(p), &sinp, &cosp);
-    temp = sqrtf(1.0f - p->dz * p->dz);
-    if (temp == 0.0f) {
+    float b = 1.0f - p->dz * p->dz;
+    if (b == 0.0f) { /* travelling along z */
+        float sint = sqrtf(a);
         p->dx = sint * cosp;
         p->dy = sint * sinp;
         p->dz = copysignf(cost, p->dz * cost);
     } else {
-        float cxp = (sint * (p->dx * p->dz * cosp - p->dy * sinp)) / temp + p->dx * cost;
-        float cyp = (sint * (p->dy * p->dz * cosp + p->dx * sinp)) / temp + p->dy * cost;
-        p->dz = -sint * cosp * temp + p->dz * cost;
+        float P = sqrtf(a * b);
+        float Q = P / b;
+        float cxp = Q * (p->dx * p->dz * cosp - p->dy * sinp) + p->dx * cost;
+        float cyp = Q * (p->dy * p->dz * cosp + p->dx * sinp) + p->dy * cost;
+        p->dz = -P * cosp + p->dz * cost;
         p->dx = cxp;
         p->dy = cyp;
     }
-    temp = 1.0f / sqrtf(p->dx * p->dx + p->dy * p->dy + p->dz * p->dz);
-    p->dx = p->dx * temp;
-    p->dy = p->dy * temp;
-    p->dz = p->dz * temp;
+    float d = p->dx * p->dx + p->dy * p->dy + p->dz * p->dz - 1.0f;
+    float r = fmaf(d, fmaf(d, 0.375f, -0.5f), 1.0f);
+    p->dx = p->dx * r;
+    p->dy = p->dy * r;
+    p->dz = p->dz * r;
     return cost;
 }
 
