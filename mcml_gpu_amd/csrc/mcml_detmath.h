@@ -126,6 +126,31 @@ MCML_HD float mcml_log(float x)
     return fmaf(fe, MCML_LN2_HI, fmaf(fe, MCML_LN2_LO, lm));
 }
 
+/* log(u) for the generator's outputs u in [2^-24, 1] (never subnormal): the same
+ * arithmetic as mcml_log without the subnormal rescaling. */
+MCML_HD float mcml_log_unit(float x)
+{
+    uint32_t ix = mcml_f2u(x);
+    int32_t e = (int32_t)(ix >> 23) - 127;
+    uint32_t mb = (ix & 0x007FFFFFu) | 0x3F800000u;
+    if (mb > 0x3FB504F3u) {
+        mb -= 0x00800000u;
+        e += 1;
+    }
+    float f = mcml_u2f(mb) - 1.0f;
+    float p = fmaf(f, -7.623060793e-02f, 1.290550232e-01f);
+    p = fmaf(f, p, -1.324298084e-01f);
+    p = fmaf(f, p, 1.418163478e-01f);
+    p = fmaf(f, p, -1.660905629e-01f);
+    p = fmaf(f, p, 2.000198513e-01f);
+    p = fmaf(f, p, -2.500158548e-01f);
+    p = fmaf(f, p, 3.333332539e-01f);
+    p = fmaf(f, p, -4.999998808e-01f);
+    float lm = fmaf(f * f, p, f);
+    float fe = (float)e;
+    return fmaf(fe, MCML_LN2_HI, fmaf(fe, MCML_LN2_LO, lm));
+}
+
 /* ---------------------------------------------------------------- sin / cos */
 #define MCML_TWO_PI 6.28318530717958647692f
 #define MCML_INV_TWO_PI 0.159154943091895335768f
@@ -147,11 +172,13 @@ MCML_HD void mcml_sincos_turns(float t, float* sn, float* cs)
     pc = fmaf(z, pc, 4.166666667e-02f);                        /* 1/4! */
     pc = fmaf(z, pc, -0.5f);                                   /* -1/2! */
     float c = fmaf(z, pc, 1.0f);
-    float rs, rc;
-    if (k == 0) { rs = s; rc = c; }
-    else if (k == 1) { rs = c; rc = -s; }
-    else if (k == 2) { rs = -s; rc = -c; }
-    else { rs = -c; rc = s; }
+    /* quadrant k: (sin, cos) = (s, c), (c, -s), (-s, -c), (-c, s) -- as selects and sign
+     * flips, with no data-dependent branch */
+    const int32_t odd = k & 1;
+    float rs = odd ? c : s;
+    float rc = odd ? s : c;
+    rs = mcml_u2f(mcml_f2u(rs) ^ ((uint32_t)(k & 2) << 30));
+    rc = mcml_u2f(mcml_f2u(rc) ^ ((uint32_t)((k + 1) & 2) << 30));
     *sn = rs;
     *cs = rc;
 }

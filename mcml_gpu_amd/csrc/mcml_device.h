@@ -8,10 +8,18 @@
 #include <stdint.h>
 #include "../../include/mcml.h"
 
-/* One layer of one parameter set, 32 B so a lane fetches it with two 16-B loads.  Index
- * 0 is the upper ambient medium, 1..L tissue, L+1 the lower ambient medium. */
-struct DevLayer {
-    float z_min, z_max, mutr, mua, g, n, pad0, pad1;
+/* One layer of one parameter set.  Index 0 is the upper ambient medium, 1..L tissue,
+ * L+1 the lower ambient medium (only n is used for the ambient entries).  The first 48 B
+ * are what a photon keeps in registers while it stays in the layer; the boundary block
+ * holds, for the interface toward layer-1 ("up") and layer+1 ("dn"), the refractive
+ * index beyond it and the ratios the Fresnel test needs.  The host forms every entry with
+ * the same float expression the transport would evaluate inline. */
+struct alignas(16) DevLayer {
+    float z_min, z_max, mutr, albedo;  /* albedo = mua * mutr */
+    float g, hg_a, hg_b, hg_c;         /* 1 - g*g, 1 - g, 2*g */
+    float hg_d, hg_inv2g, n, mua;      /* 1 + g*g, 1 / (2*g) */
+    float up_n, up_ratio, up_ratio2, up_rnorm; /* n2, n1/n2, (n1*n1)/(n2*n2), (n1-n2)/(n1+n2) */
+    float dn_n, dn_ratio, dn_ratio2, dn_rnorm;
 };
 
 /* One detector (index 1..S; entry 0 unused), 32 B. */
@@ -21,7 +29,7 @@ struct DevDet {
 
 /* One tissue parameter set with its photon budget.  Photon k of set j is global queue
  * ticket ticket_begin + k. */
-struct DevSet {
+struct alignas(16) DevSet {
     DevLayer layer[MCML_MAX_LAYERS + 2];
     DevDet det[MCML_MAX_SDS + 1];
     uint64_t ticket_begin;

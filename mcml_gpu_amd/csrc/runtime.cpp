@@ -119,12 +119,32 @@ void fill_devset(const mcml_derived& d, DevSet& S)
 {
     memset(&S, 0, sizeof(S));
     for (int l = 0; l < d.num_layers + 2; l++) {
-        S.layer[l].z_min = d.layer[l].z_min;
-        S.layer[l].z_max = d.layer[l].z_max;
-        S.layer[l].mutr = d.layer[l].mutr;
-        S.layer[l].mua = d.layer[l].mua;
-        S.layer[l].g = d.layer[l].g;
-        S.layer[l].n = d.layer[l].n;
+        const mcml_layer_derived& in = d.layer[l];
+        DevLayer& o = S.layer[l];
+        o.z_min = in.z_min;
+        o.z_max = in.z_max;
+        o.mutr = in.mutr;
+        o.mua = in.mua;
+        o.albedo = in.mua * in.mutr;
+        o.g = in.g;
+        o.hg_a = 1.0f - in.g * in.g;
+        o.hg_b = 1.0f - in.g;
+        o.hg_c = 2.0f * in.g;
+        o.hg_d = 1.0f + in.g * in.g;
+        o.hg_inv2g = 1.0f / (2.0f * in.g);
+        o.n = in.n;
+        if (l >= 1 && l <= d.num_layers) {
+            const float n1 = in.n;
+            const float nu = d.layer[l - 1].n, nd = d.layer[l + 1].n;
+            o.up_n = nu;
+            o.up_ratio = n1 / nu;
+            o.up_ratio2 = (n1 * n1) / (nu * nu);
+            o.up_rnorm = (n1 - nu) / (n1 + nu);
+            o.dn_n = nd;
+            o.dn_ratio = n1 / nd;
+            o.dn_ratio2 = (n1 * n1) / (nd * nd);
+            o.dn_rnorm = (n1 - nd) / (n1 + nd);
+        }
     }
     for (int i = 1; i <= d.num_sds; i++) {
         S.det[i].pos = d.det_pos[i];

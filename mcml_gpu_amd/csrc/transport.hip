@@ -62,24 +62,25 @@ struct Photon {
     int32_t steps;
     int32_t first;
     mcml_mwc rng;
-    /* the current layer's parameters, reloaded only when the layer changes; the
-     * Henyey-Greenstein constants are the same float expressions the step would form */
-    float zmin, zmax, mutr, albedo, g, hg_a, hg_b, hg_c, hg_d, hg_inv2g;
+    /* the current layer's parameters, reloaded only when the layer changes */
+    float zmin, zmax, mutr, albedo, g, hg_a, hg_b, hg_c, hg_d, hg_inv2g, n;
 };
 
 __device__ __forceinline__ void load_layer(Photon& p, const DevSet* S)
 {
-    const DevLayer L = S->layer[p.layer];
-    p.zmin = L.z_min;
-    p.zmax = L.z_max;
-    p.mutr = L.mutr;
-    p.albedo = L.mua * L.mutr;
-    p.g = L.g;
-    p.hg_a = 1.0f - L.g * L.g;
-    p.hg_b = 1.0f - L.g;
-    p.hg_c = 2.0f * L.g;
-    p.hg_d = 1.0f + L.g * L.g;
-    p.hg_inv2g = 1.0f / (2.0f * L.g);
+    const float4* L = reinterpret_cast<const float4*>(&S->layer[p.layer]);
+    const float4 a = L[0], b = L[1], c = L[2];
+    p.zmin = a.x;
+    p.zmax = a.y;
+    p.mutr = a.z;
+    p.albedo = a.w;
+    p.g = b.x;
+    p.hg_a = b.y;
+    p.hg_b = b.z;
+    p.hg_c = b.w;
+    p.hg_d = c.x;
+    p.hg_inv2g = c.y;
+    p.n = c.z;
 }
 
 /* Launch photon k of set `set` (LaunchPhoton, transport.cu:624-719). */
@@ -162,11 +163,15 @@ __device__ __forceinline__ void spin(Photon& p)
     p.dz = p.dz * r;
 }
 
-/* Fresnel boundary (Reflect, transport.cu:790-853): true = reflected. */
+/* Fresnel boundary (Reflect, transport.cu:790-853): true = reflected.  The interface
+ * constants come precomputed from the layer table (same float expressions). */
 __device__ __forceinline__ bool reflect(Photon& p, const DevSet* S, int new_layer)
 {
-    const float n1 = S->layer[p.layer].n;
-    const float n2 = S->layer[new_layer].n;
+    const DevLayer& Lc = S->layer[p.layer];
+    const bool up = new_layer < p.layer;
+    const float4 B = reinterpret_cast<const float4*>(&Lc)[up ? 3 : 4];
+    const float n1 = p.n;
+    const float n2 = B.x;
     const float ci = fabsf(p.dz);
     float r;
     if (n1 == n2) {
@@ -178,7 +183,7 @@ __device__ __forceinline__ bool reflect(Photon& p, const DevSet* S, int new_laye
         return true;
     }
     if (ci == 1.0f) {
-        r = (n1 - n2) / (n1 + n2);
+        r = B.w;
         if (mcml_uniform(&p.rng) <= r * r) {
             p.dz = -p.dz;
             return true;
@@ -186,7 +191,7 @@ __device__ __forceinline__ bool reflect(Photon& p, const DevSet* S, int new_laye
         p.layer = new_layer;
         return false;
     }
-    float e = (n1 * n1) / (n2 * n2) * (1.0f - ci * ci);
+    float e = B.z * (1.0f - ci * ci);
     r = 2.0f * sqrtf((1.0f - ci * ci) * (1.0f - e) * e * ci * ci);
     e = e + (ci * ci) * (1.0f - 2.0f * e);
     r = e * ((1.0f - e - r) / ((1.0f - e + r) * (e + r)));
@@ -194,7 +199,7 @@ __device__ __forceinline__ bool reflect(Photon& p, const DevSet* S, int new_laye
         p.dz = -p.dz;
         return true;
     }
-    r = n1 / n2;
+    r = B.y;
     e = r * r * (1.0f - ci * ci);
     p.dx = p.dx * r;
     p.dy = p.dy * r;
@@ -245,7 +250,7 @@ __device__ __forceinline__ int hop_drop_spin(Photon& p, const DevSet* S, int tar
                                              const ReplayArgs* ra, uint64_t& rl_idx, uint64_t& rl_w,
                                              uint32_t& n_bnd, uint32_t& n_int)
 {
-    float s = -mcml_log(mcml_uniform(&p.rng)) * p.mutr;
+    float s = -mcml_log_unit(mcml_uniform(&p.rng)) * p.mutr;
     int new_layer = p.layer;
     if (p.z + s * p.dz < p.zmin) {
         new_layer--;

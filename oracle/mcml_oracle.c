@@ -320,7 +320,7 @@ uint32_t oracle_photon_life(const mcml_derived* d, oracle_photon* p, int target,
     uint32_t detected = 0;
     for (;;) {
         const mcml_layer_derived* Ly = &d->layer[p->layer];
-        float s = -mcml_log(urand(p)) * Ly->mutr; /* transport.cu:373 */
+        float s = -mcml_log_unit(urand(p)) * Ly->mutr; /* transport.cu:373 */
         int new_layer = p->layer;
         if (p->z + s * p->dz < Ly->z_min) { /* transport.cu:379-386 */
             new_layer--;
@@ -504,6 +504,10 @@ void oracle_log_n(const float* x, float* y, int64_t n)
 {
     for (int64_t i = 0; i < n; i++) y[i] = mcml_log(x[i]);
 }
+void oracle_log_unit_n(const float* x, float* y, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) y[i] = mcml_log_unit(x[i]);
+}
 void oracle_sincos_turns_n(const float* t, float* s, float* c, int64_t n)
 {
     for (int64_t i = 0; i < n; i++) mcml_sincos_turns(t[i], &s[i], &c[i]);
@@ -573,7 +577,7 @@ void oracle_steps_n(const mcml_derived* d, int layer, uint64_t seed, float* out,
     oracle_photon p;
     memset(&p, 0, sizeof(p));
     p.rng = mcml_stream(seed, 0, 3, mcml_mwc_multipliers, MCML_MWC_TABLE_SIZE);
-    for (int64_t i = 0; i < n; i++) out[i] = -mcml_log(urand(&p)) * d->layer[layer].mutr;
+    for (int64_t i = 0; i < n; i++) out[i] = -mcml_log_unit(urand(&p)) * d->layer[layer].mutr;
 }
 /* Detection test on an explicit exit state (x, y, dx, dy, dz, w); returns the mask. */
 uint32_t oracle_detect_state(const mcml_derived* d, const float* st, int only, float* wout)

@@ -63,11 +63,12 @@ def hemisphere(photons, mua=10.0, mus=90.0, g=0.0, n=1.0):
     return abi.make_sim(cfg, [(0.0, mua, mus, n, g)])
 
 
-def vdh_slab(photons):
-    """van de Hulst slab: d = 0.02 cm, mua 10, mus 90, g 0.75, n 1.5 in air, realised as a
-    2-layer model whose second layer is a clear index-1 medium (mus tiny, mua 0)."""
+def vdh_slab(photons, n_slab=1.0):
+    """van de Hulst's slab: d = 0.02 cm, mua 10, mus 90, g 0.75, index n_slab in air,
+    realised as a 2-layer model whose second layer is a clear index-1 medium (mus tiny,
+    mua 0) leading to the lower ambient medium."""
     cfg = base_cfg(photons, 2, 1.0, 1.0, 1, 1, probes((0, 0, 0), [(0.0, 1.0, 1000.0, 0.0)]))
-    return abi.make_sim(cfg, [(0.02, 10.0, 90.0, 1.5, 0.75), (0.0, 0.0, 1e-6, 1.0, 0.0)])
+    return abi.make_sim(cfg, [(0.02, 10.0, 90.0, n_slab, 0.75), (0.0, 0.0, 1e-6, 1.0, 0.0)])
 
 
 def fiber_busy(photons):

@@ -44,6 +44,7 @@ def lib():
     L.oracle_multiplier_table.argtypes = [C.POINTER(C.c_uint32)]
     L.oracle_multiplier_table.restype = C.POINTER(C.c_uint32)
     L.oracle_log_n.argtypes = [_fp, _fp, C.c_int64]
+    L.oracle_log_unit_n.argtypes = [_fp, _fp, C.c_int64]
     L.oracle_sincos_turns_n.argtypes = [_fp, _fp, _fp, C.c_int64]
     L.oracle_acos_n.argtypes = [_fp, _fp, C.c_int64]
     L.oracle_spin_n.argtypes = [C.c_float, _fp, C.c_uint64, _fp, _fp, C.c_int64]

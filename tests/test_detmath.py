@@ -1,0 +1,74 @@
+"""The deterministic math of mcml_detmath.h against float64 references (CPU).
+
+These pin the functions both the oracle and the HIP kernels use: a wrong coefficient,
+sign or quadrant fails here, independently of any transport test."""
+import numpy as np
+
+import oracle_ffi as O
+
+
+def ulp_err(y32, ref64):
+    y = y32.astype(np.float64)
+    sp = np.spacing(np.abs(ref64).astype(np.float32)).astype(np.float64)
+    return np.abs(y - ref64) / sp
+
+
+def test_log_all_generator_outputs():
+    # every value the generator can produce: k * 2^-24, k = 1 .. 2^24
+    k = np.arange(1, 2 ** 24 + 1, dtype=np.float64)
+    u = (k * 2.0 ** -24).astype(np.float32)
+    y = np.zeros_like(u)
+    O.lib().oracle_log_unit_n(u, y, len(u))
+    ref = np.log(u.astype(np.float64))
+    nz = ref != 0
+    assert ulp_err(y[nz], ref[nz]).max() < 1.5
+    assert y[-1] == 0.0  # log(1) = 0 exactly: u = 1 gives a zero-length step
+    y2 = np.zeros_like(u)
+    O.lib().oracle_log_n(u, y2, len(u))
+    assert np.array_equal(y.view(np.uint32), y2.view(np.uint32))
+
+
+def test_log_general_and_subnormal():
+    rng = np.random.default_rng(3)
+    x = np.concatenate([
+        rng.uniform(1e-3, 1e3, 200000), 10.0 ** rng.uniform(-37, 38, 200000),
+        np.array([1.0, 2.0, 0.5, np.sqrt(2.0), 1.4142135, 1.4142137]),
+        np.array([1e-40, 1e-45, 3e-39, 1.1754942e-38])]).astype(np.float32)
+    y = np.zeros_like(x)
+    O.lib().oracle_log_n(x, y, len(x))
+    ref = np.log(x.astype(np.float64))
+    nz = ref != 0
+    assert ulp_err(y[nz], ref[nz]).max() < 1.5
+
+
+def test_sincos_turns():
+    rng = np.random.default_rng(4)
+    t = np.concatenate([rng.uniform(0, 1, 1000000), np.arange(0, 1.0001, 1 / 64)]).astype(np.float32)
+    s = np.zeros_like(t)
+    c = np.zeros_like(t)
+    O.lib().oracle_sincos_turns_n(t, s, c, len(t))
+    a = 2 * np.pi * t.astype(np.float64)
+    assert np.abs(s - np.sin(a)).max() < 1.2e-7
+    assert np.abs(c - np.cos(a)).max() < 1.2e-7
+    # quarter turns are exact
+    q = np.array([0, 0.25, 0.5, 0.75, 1.0], np.float32)
+    s = np.zeros_like(q)
+    c = np.zeros_like(q)
+    O.lib().oracle_sincos_turns_n(q, s, c, len(q))
+    assert list(np.abs(s)) == [0, 1, 0, 1, 0] and list(np.abs(c)) == [1, 0, 1, 0, 1]
+    assert list(np.sign(s)[[1, 3]]) == [1, -1] and list(np.sign(c)[[0, 2, 4]]) == [1, -1, 1]
+
+
+def test_acos():
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(-1, 1, 1000000),
+                        np.array([-1, -0.5, -0.4999999, 0, 0.5, 0.5000001, 1], np.float64)]).astype(np.float32)
+    y = np.zeros_like(x)
+    O.lib().oracle_acos_n(x, y, len(x))
+    ref = np.arccos(x.astype(np.float64))
+    assert np.abs(y - ref).max() < 3e-7
+    assert y[-1] == 0.0
+    bad = np.array([1.0000001, -1.0000001, np.inf, np.nan], np.float32)
+    yb = np.zeros_like(bad)
+    O.lib().oracle_acos_n(bad, yb, len(bad))
+    assert np.isnan(yb).all()
