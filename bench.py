@@ -44,6 +44,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--bpc", type=int, default=0)
+    ap.add_argument("--bnd", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=float, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter scale)")
     args = ap.parse_args()
@@ -70,7 +71,8 @@ def main():
     sim = M.load_config(args.config)
     sim.number_photons = int(args.photons)
     opts = M.make_opts(replay=True, pathlength=True, binary=True, block_size=args.block,
-                       chunk=args.chunk, refill_min=args.refill, blocks_per_cu=args.bpc)
+                       chunk=args.chunk, refill_min=args.refill, blocks_per_cu=args.bpc,
+                       boundary_min=args.bnd)
     ctx = M.Context(0)
     for _ in range(args.warmup):
         ctx.run(sim, opts, seed=args.seed)
@@ -113,7 +115,7 @@ def main():
             "layers": sim.number_layers,
             "sds": sim.num_sds,
             "block": res.block_size, "grid": res.grid_size, "chunk": res.chunk,
-            "refill_min": res.refill_min,
+            "refill_min": res.refill_min, "boundary_min": res.boundary_min,
         },
         "kernel_photons_per_s": n / (k_ms * 1e-3),
         "first_pass_kernel_ms": k_ms,

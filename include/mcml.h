@@ -133,6 +133,8 @@ typedef struct {
     int32_t blocks_per_cu;   /* resident workgroups per CU; 0 = occupancy maximum */
     int32_t chunk;           /* photon ids a wavefront claims per queue access; 0 = tuned */
     int32_t refill_min;      /* refill when at least this many lanes are idle; 0 = tuned */
+    int32_t boundary_min;    /* resolve boundary hits when at least this many lanes wait on
+                                one; 0 = tuned (results never depend on the launch knobs) */
     int32_t reference_shape; /* 1 = emulate the reference launch shape (32-lane groups,
                                 one photon per lane, 50000-step relaunch with host sync) */
 } mcml_opts;
@@ -177,7 +179,7 @@ typedef struct {
     double ms_first_pass_wall;
     double ms_replay_wall;
     int32_t first_pass_launches; /* kernel launches the first pass needed */
-    int32_t block_size, grid_size, chunk, refill_min; /* launch shape used */
+    int32_t block_size, grid_size, chunk, refill_min, boundary_min; /* launch shape used */
 } mcml_result;
 
 typedef struct mcml_ctx mcml_ctx;

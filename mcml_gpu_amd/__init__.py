@@ -90,9 +90,9 @@ def load_config(name):
 
 
 def make_opts(replay=False, absorbance=False, pathlength=False, avg_pathlength=False, binary=False,
-              block_size=0, blocks_per_cu=0, chunk=0, refill_min=0, reference_shape=0):
+              block_size=0, blocks_per_cu=0, chunk=0, refill_min=0, boundary_min=0, reference_shape=0):
     return abi.Opts(int(replay), int(absorbance), int(pathlength), int(avg_pathlength), int(binary),
-                    block_size, blocks_per_cu, chunk, refill_min, reference_shape)
+                    block_size, blocks_per_cu, chunk, refill_min, boundary_min, reference_shape)
 
 
 class RunResult:
@@ -127,7 +127,7 @@ class RunResult:
                 self.A_rz = self.A0_z = None
         for f in ("photons", "steps", "steps_boundary", "steps_interaction", "ms_first_pass_kernel",
                   "ms_replay_kernel", "ms_dump_kernel", "ms_first_pass_wall", "ms_replay_wall",
-                  "first_pass_launches", "block_size", "grid_size", "chunk", "refill_min"):
+                  "first_pass_launches", "block_size", "grid_size", "chunk", "refill_min", "boundary_min"):
             setattr(self, f, getattr(r, f))
 
     def pathlength_records(self, set_index, sds):

@@ -93,7 +93,7 @@ class Derived(C.Structure):
 class Opts(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "replay", "absorbance", "pathlength", "avg_pathlength", "binary",
-        "block_size", "blocks_per_cu", "chunk", "refill_min", "reference_shape")]
+        "block_size", "blocks_per_cu", "chunk", "refill_min", "boundary_min", "reference_shape")]
 
 
 class Record(C.Structure):
@@ -131,6 +131,7 @@ class Result(C.Structure):
         ("grid_size", C.c_int32),
         ("chunk", C.c_int32),
         ("refill_min", C.c_int32),
+        ("boundary_min", C.c_int32),
     ]
 
 

@@ -38,7 +38,11 @@ struct alignas(16) DevSet {
     float start_weight, det_reflectance;
     int32_t src_oblique, det_oblique;
     float src_radius, src_na, src_theta_max, src_sin_a, src_cos_a;
-    int32_t pad;
+    /* conservative reject box of all detectors (never rejects an accepted photon):
+     * fiber mode: y*y > y2max, or x outside [xlo, xhi] (xlo > xhi disables the x test);
+     * ring mode: dist outside [dlo, dhi] or |dz| < cosmin */
+    float y2max, xlo, xhi, dlo, dhi, cosmin;
+    int32_t pad[2];
 };
 
 /* Work counters accumulated by the first-pass kernel (one atomic per lane at exit). */
@@ -50,6 +54,8 @@ struct FirstPassArgs {
     int32_t nsets;
     int32_t chunk;             /* tickets a wavefront claims per queue access (>= 64) */
     int32_t refill_min;        /* refill once this many lanes of the wavefront are idle */
+    int32_t bnd_min;           /* resolve boundary hits once this many lanes wait on one */
+    int32_t lds_sets;          /* 1: stage the set table in LDS (dynamic shared memory) */
     int32_t pad;
     uint64_t total;            /* total tickets */
     uint64_t seed;
@@ -77,6 +83,8 @@ struct ReplayArgs {
     unsigned long long* queue;
     int32_t chunk;
     int32_t num_layers;
+    int32_t bnd_min;
+    int32_t pad2;
     /* outputs, structure of arrays over jobs */
     float* weight;             /* [njobs] detected weight (0 if not re-detected) */
     float* steps;              /* [njobs] step count as float */

@@ -30,6 +30,11 @@
 namespace {
 
 constexpr float kPI = 3.141592654f; /* header.h:40 */
+/* launch defaults (results never depend on them: each photon's stream is fixed) */
+constexpr int kDefaultChunk = 128;
+constexpr int kDefaultRefill = 2;
+constexpr int kDefaultBoundary = 1;
+constexpr size_t kLdsTableMax = 16384;
 
 struct DevBuf {
     void* p = nullptr;
@@ -166,6 +171,32 @@ void fill_devset(const mcml_derived& d, DevSet& S)
     S.src_theta_max = d.src_theta_max;
     S.src_sin_a = d.src_sin_a;
     S.src_cos_a = d.src_cos_a;
+    /* detector reject box; the x extent of a tilted fibre's footprint is r / cos(angle),
+     * widened by 1% so float rounding can never move an accepted photon outside it */
+    S.y2max = 0.0f;
+    S.xlo = 3.0e38f;
+    S.xhi = -3.0e38f;
+    S.dlo = 3.0e38f;
+    S.dhi = -3.0e38f;
+    S.cosmin = 3.0e38f;
+    bool x_ok = true;
+    for (int i = 1; i <= d.num_sds; i++) {
+        S.y2max = std::max(S.y2max, d.det_r2[i]);
+        const float ca = fabsf(d.det_cos_a[i]);
+        if (ca < 0.01f) x_ok = false;
+        else {
+            const float half = 1.01f * d.det_radius[i] / ca + 1e-6f;
+            S.xlo = std::min(S.xlo, d.det_pos[i] - half);
+            S.xhi = std::max(S.xhi, d.det_pos[i] + half);
+        }
+        S.dlo = std::min(S.dlo, d.det_pos[i] - d.det_radius[i]);
+        S.dhi = std::max(S.dhi, d.det_pos[i] + d.det_radius[i]);
+        S.cosmin = std::min(S.cosmin, d.det_cos_crit[i]);
+    }
+    if (!x_ok) {
+        S.xlo = 1.0f;
+        S.xhi = -1.0f;
+    }
 }
 
 bool rec_less(const mcml_record& a, const mcml_record& b)
@@ -377,13 +408,18 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
         set_err(c, "block_size must be a multiple of 64 and at most 256");
         return MCML_ERR_ARG;
     }
+    /* a small set table is staged in LDS by every workgroup; a large batch reads it from
+     * global memory (it is only read at launch, layer changes and boundary hits) */
+    const size_t table_bytes = sizeof(DevSet) * (size_t)nsets;
+    const int lds_sets = table_bytes <= kLdsTableMax ? 1 : 0;
     int occ = 0;
-    HIP_TRY(c, mcml_occupancy_first_pass(block, &occ));
+    HIP_TRY(c, mcml_occupancy_first_pass(block, lds_sets ? table_bytes : 0, &occ));
     if (occ < 1) occ = 1;
     const int bpc = o.blocks_per_cu > 0 ? std::min(o.blocks_per_cu, occ) : occ;
     const int grid = c->cu_count * bpc;
-    const int chunk = o.chunk > 0 ? std::max(o.chunk, 64) : 256;
-    const int refill_min = o.refill_min > 0 ? std::min(o.refill_min, 64) : 1;
+    const int chunk = o.chunk > 0 ? std::max(o.chunk, 64) : kDefaultChunk;
+    const int refill_min = o.refill_min > 0 ? std::min(o.refill_min, 64) : kDefaultRefill;
+    const int bnd_min = o.boundary_min > 0 ? std::min(o.boundary_min, 64) : kDefaultBoundary;
 
     /* record capacity: detection is rare for small fibers; grow and rerun on overflow
      * (the result does not depend on the capacity, each photon's stream is fixed) */
@@ -404,6 +440,8 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
         a.nsets = nsets;
         a.chunk = chunk;
         a.refill_min = refill_min;
+        a.bnd_min = bnd_min;
+        a.lds_sets = lds_sets;
         a.total = total;
         a.seed = seed;
         a.mult = c->d_mult;
@@ -439,6 +477,7 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
     out->grid_size = grid;
     out->chunk = chunk;
     out->refill_min = refill_min;
+    out->boundary_min = bnd_min;
     if (out->photons != total) {
         set_err(c, "first pass launched %llu photons, expected %llu", (unsigned long long)out->photons,
                 (unsigned long long)total);
@@ -506,6 +545,7 @@ int replay(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opts& o
         a.queue = as<unsigned long long>(c->queue);
         a.chunk = 64;
         a.num_layers = L;
+        a.bnd_min = o.boundary_min > 0 ? std::min(o.boundary_min, 64) : kDefaultBoundary;
         a.weight = as<float>(c->rw);
         a.steps = as<float>(c->rs);
         a.pl = as<float>(c->rpl);

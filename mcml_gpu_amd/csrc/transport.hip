@@ -208,24 +208,36 @@ __device__ __forceinline__ bool reflect(Photon& p, const DevSet* S, int new_laye
     return false;
 }
 
-/* Does detector i accept the escaping photon (detect / detect_replay, transport.cu:872-1000)?
- * Fiber mode: tilted elliptical footprint + NA cone; ring mode: annulus + NA cone. */
-__device__ __forceinline__ bool accepts(const Photon& p, const DevSet* S, int i)
+/* Detectors accepting the escaping photon (detect / detect_replay, transport.cu:872-1000),
+ * as a bit mask (bit i = SDS i).  Fiber mode: tilted elliptical footprint + NA cone; ring
+ * mode: annulus + NA cone.  `target` = 0 tests every detector, otherwise that one only.
+ * The set's reject box skips the per-detector tests for photons no detector can accept. */
+__device__ __forceinline__ uint32_t detect_mask(const Photon& p, const DevSet* S, int target)
 {
-    const DevDet D = S->det[i];
+    const int i0 = target ? target : 1, i1 = target ? target : S->num_sds;
+    uint32_t m = 0;
     if (S->det_oblique) {
-        const float t = (p.x - D.pos) * D.cos_a;
-        if (t * t + p.y * p.y <= D.r2) {
-            const float uz = p.dx * D.sin_a + fabsf(p.dz * D.cos_a);
-            return fabsf(uz) >= D.cos_crit;
+        if (p.y * p.y > S->y2max) return 0u;
+        if (S->xlo <= S->xhi && (p.x < S->xlo || p.x > S->xhi)) return 0u;
+        for (int i = i0; i <= i1; i++) {
+            const DevDet D = S->det[i];
+            const float t = (p.x - D.pos) * D.cos_a;
+            if (t * t + p.y * p.y <= D.r2) {
+                const float uz = p.dx * D.sin_a + fabsf(p.dz * D.cos_a);
+                if (fabsf(uz) >= D.cos_crit) m |= 1u << i;
+            }
         }
-        return false;
+        return m;
     }
-    if (fabsf(p.dz) >= D.cos_crit) {
-        const float dist = sqrtf(p.x * p.x + p.y * p.y);
-        return dist >= D.pos - D.radius && dist <= D.pos + D.radius;
+    const float adz = fabsf(p.dz);
+    if (adz < S->cosmin) return 0u;
+    const float dist = sqrtf(p.x * p.x + p.y * p.y);
+    if (dist < S->dlo || dist > S->dhi) return 0u;
+    for (int i = i0; i <= i1; i++) {
+        const DevDet D = S->det[i];
+        if (adz >= D.cos_crit && dist >= D.pos - D.radius && dist <= D.pos + D.radius) m |= 1u << i;
     }
-    return false;
+    return m;
 }
 
 /* Weight credited to detector i (ring mode applies the arc fraction acos(t)/pi). */
@@ -241,89 +253,36 @@ __device__ __forceinline__ float credited(const Photon& p, const DevSet* S, int 
 
 enum StepOutcome { ST_ALIVE = 0, ST_DEAD = 1, ST_DETECTED = 2 };
 
-/* Hop, boundary, drop, spin for one step; returns the outcome.  `target` = 0 tests every
- * detector (first pass), otherwise only that detector (replay).  On ST_DETECTED, *mask
- * holds the accepting detectors (bit i = SDS i). */
+/* Hop (transport.cu:372-395): sample the step, stop it at the layer boundary, move, count
+ * the step, clamp z.  Returns true when the hop ended on a boundary (*nl = layer beyond). */
 template <bool REPLAY>
-__device__ __forceinline__ int hop_drop_spin(Photon& p, const DevSet* S, int target, uint32_t* mask, float* pl_lane,
-                                             int pl_stride, unsigned long long* a_rz, unsigned long long* a0_z,
-                                             const ReplayArgs* ra, uint64_t& rl_idx, uint64_t& rl_w,
-                                             uint32_t& n_bnd, uint32_t& n_int)
+__device__ __forceinline__ bool hop(Photon& p, float& s, int& nl, float* pl_lane, int pl_stride)
 {
-    float s = -mcml_log_unit(mcml_uniform(&p.rng)) * p.mutr;
-    int new_layer = p.layer;
+    s = -mcml_log_unit(mcml_uniform(&p.rng)) * p.mutr;
+    nl = p.layer;
     if (p.z + s * p.dz < p.zmin) {
-        new_layer--;
+        nl--;
         s = (p.zmin - p.z) / p.dz;
     }
     if (p.z + s * p.dz > p.zmax) {
-        new_layer++;
+        nl++;
         s = (p.zmax - p.z) / p.dz;
     }
     p.x += p.dx * s;
     p.y += p.dy * s;
     p.z += p.dz * s;
     p.steps++;
-    if (REPLAY) pl_lane[(p.layer - 1) * pl_stride] += s;
+    if (REPLAY) pl_lane[(p.layer - 1) * pl_stride] += s; /* transport.cu:529 */
     if (p.z > p.zmax) p.z = p.zmax;
     if (p.z < p.zmin) p.z = p.zmin;
+    return nl != p.layer;
+}
 
-    if (new_layer != p.layer) {
-        n_bnd++;
-        const int temp_layer = p.layer;
-        if (!reflect(p, S, new_layer)) {
-            if (new_layer == 0) {
-                uint32_t m = 0;
-                if (REPLAY) {
-                    if (accepts(p, S, target)) m = 1u << target;
-                } else {
-                    for (int i = 1; i <= S->num_sds; i++)
-                        if (accepts(p, S, i)) m |= 1u << i;
-                }
-                if (m) {
-                    *mask = m;
-                    return ST_DETECTED;
-                }
-                if (mcml_uniform(&p.rng) > S->det_reflectance) return ST_DEAD;
-                p.dz = -p.dz; /* the detector reflects the photon back, direction kept */
-                p.z = -p.z;
-                p.layer = temp_layer;
-            } else if (new_layer > S->num_layers) {
-                return ST_DEAD;
-            } else {
-                load_layer(p, S);
-            }
-        }
-        /* a boundary step ends with s = 0: no drop, no spin */
-    } else if (s > 0.0f) {
-        n_int++;
-        const float w_temp = rintf(p.albedo * p.w);
-        p.w -= w_temp;
-        if (REPLAY && a_rz != nullptr) {
-            float qz = p.z / ra->dz;
-            qz = fminf(fmaxf(qz, 0.0f), (float)(ra->nz - 1));
-            const int zb = (int)qz;
-            if (p.first) {
-                atomicAdd(&a0_z[zb], (unsigned long long)w_temp);
-            } else {
-                float qr = sqrtf(p.x * p.x + p.y * p.y) / ra->dr;
-                qr = fminf(fmaxf(qr, 0.0f), (float)(ra->nr - 1));
-                const uint64_t idx = (uint64_t)zb * (uint64_t)ra->nr + (uint64_t)(int)qr;
-                if (idx == rl_idx) {
-                    rl_w += (uint64_t)w_temp;
-                } else {
-                    if (rl_w != 0) atomicAdd(&a_rz[rl_idx], (unsigned long long)rl_w);
-                    rl_idx = idx;
-                    rl_w = (uint64_t)w_temp;
-                }
-            }
-        }
-        p.first = 0;
-        spin(p);
-    }
-    /* PhotonSurvive (transport.cu:855-870).  The reference applies the step cap in the
-     * first pass only; a replayed photon was detected within the cap, so applying it in
-     * replay as well changes nothing for detected photons and bounds every replay loop. */
+/* PhotonSurvive (transport.cu:855-870).  The reference applies the step cap in the first
+ * pass only; a replayed photon was detected within the cap, so applying it in replay too
+ * changes nothing for detected photons and bounds every replay loop. */
+__device__ __forceinline__ int survive(Photon& p)
+{
     if (p.steps >= MCML_MAX_SCATTER) return ST_DEAD;
     if (p.w > (float)MCML_WEIGHTI) return ST_ALIVE;
     if (p.w == 0.0f) return ST_DEAD;
@@ -332,6 +291,77 @@ __device__ __forceinline__ int hop_drop_spin(Photon& p, const DevSet* S, int tar
         return ST_ALIVE;
     }
     return ST_DEAD;
+}
+
+/* Absorbance bookkeeping of one replayed photon (transport.cu:580-600): the first drop goes
+ * to A0_z, later ones to A_rz, merged while consecutive drops stay in one cell. */
+struct GridLane {
+    unsigned long long* a_rz;
+    unsigned long long* a0_z;
+    uint64_t rl_idx;
+    uint64_t rl_w;
+};
+
+/* Interaction (transport.cu:430-437 / 573-603): drop weight and scatter when the hop did
+ * not end on a boundary and s > 0, then the survival test. */
+template <bool REPLAY>
+__device__ __forceinline__ int interact(Photon& p, float s, GridLane& gl, const ReplayArgs* ra, uint32_t& n_int)
+{
+    if (s > 0.0f) {
+        n_int++;
+        const float w_temp = rintf(p.albedo * p.w);
+        p.w -= w_temp;
+        if (REPLAY && gl.a_rz != nullptr) {
+            float qz = p.z / ra->dz;
+            qz = fminf(fmaxf(qz, 0.0f), (float)(ra->nz - 1));
+            const int zb = (int)qz;
+            if (p.first) {
+                atomicAdd(&gl.a0_z[zb], (unsigned long long)w_temp);
+            } else {
+                float qr = sqrtf(p.x * p.x + p.y * p.y) / ra->dr;
+                qr = fminf(fmaxf(qr, 0.0f), (float)(ra->nr - 1));
+                const uint64_t idx = (uint64_t)zb * (uint64_t)ra->nr + (uint64_t)(int)qr;
+                if (idx == gl.rl_idx) {
+                    gl.rl_w += (uint64_t)w_temp;
+                } else {
+                    if (gl.rl_w != 0) atomicAdd(&gl.a_rz[gl.rl_idx], (unsigned long long)gl.rl_w);
+                    gl.rl_idx = idx;
+                    gl.rl_w = (uint64_t)w_temp;
+                }
+            }
+        }
+        p.first = 0;
+        spin(p);
+    }
+    return survive(p);
+}
+
+/* Boundary (transport.cu:397-428): Fresnel, then escape through the top (detection, or
+ * the detector face's reflection back into the tissue), loss through the bottom, or entry
+ * into the next tissue layer; then the survival test.  `target` = 0 tests every detector
+ * (first pass), otherwise that detector only (replay).  On ST_DETECTED *mask holds the
+ * accepting detectors (bit i = SDS i). */
+__device__ __forceinline__ int cross(Photon& p, int nl, const DevSet* S, int target, uint32_t& mask)
+{
+    const int temp_layer = p.layer;
+    if (!reflect(p, S, nl)) {
+        if (nl == 0) {
+            const uint32_t m = detect_mask(p, S, target);
+            if (m) {
+                mask = m;
+                return ST_DETECTED;
+            }
+            if (mcml_uniform(&p.rng) > S->det_reflectance) return ST_DEAD;
+            p.dz = -p.dz; /* the detector face reflects the photon, refracted direction kept */
+            p.z = -p.z;
+            p.layer = temp_layer;
+        } else if (nl > S->num_layers) {
+            return ST_DEAD;
+        } else {
+            load_layer(p, S);
+        }
+    }
+    return survive(p);
 }
 
 __device__ __forceinline__ uint32_t set_of_ticket(const uint64_t* set_begin, int nsets, uint64_t t)
@@ -345,57 +375,90 @@ __device__ __forceinline__ uint32_t set_of_ticket(const uint64_t* set_begin, int
     return (uint32_t)lo;
 }
 
+/* Wavefront work queue: lanes named in `need` take consecutive ids from the wavefront's
+ * private range [q_next, q_end), which is refilled `chunk` ids at a time from the global
+ * counter.  Returns this lane's id, or ~0 when it gets none. */
+struct WaveQueue {
+    uint64_t next, end;
+    bool dry;
+};
+
+__device__ __forceinline__ uint64_t take_ids(WaveQueue& q, uint64_t need, bool mine, unsigned long long* counter,
+                                             uint64_t chunk, uint64_t total)
+{
+    const int n = __popcll(need);
+    const uint32_t rank = rank_in(need);
+    const uint64_t avail = q.end - q.next;
+    uint64_t id = ~0ull;
+    if (mine && rank < avail) id = q.next + rank;
+    if ((uint64_t)n > avail) {
+        unsigned long long base = 0;
+        if (lane_id() == (uint32_t)(__ffsll((long long)__ballot(1)) - 1)) base = atomicAdd(counter, chunk);
+        base = bcast_first(base);
+        uint64_t cend = base + chunk;
+        if (cend > total) cend = total;
+        if (base >= total) {
+            q.dry = true;
+            cend = base;
+        }
+        if (mine && rank >= avail) {
+            const uint64_t t = base + (rank - avail);
+            if (t < cend) id = t;
+        }
+        const uint64_t used = base + ((uint64_t)n - avail);
+        q.next = used < cend ? used : cend;
+        q.end = cend;
+    } else {
+        q.next += (uint64_t)n;
+    }
+    return id;
+}
+
+/* Should the wavefront resolve its pending boundary hits now?  Yes once `bnd_min` lanes
+ * wait, or when every live lane waits (no lane could make progress otherwise). */
+__device__ __forceinline__ bool resolve_now(bool pending, bool alive, int bnd_min)
+{
+    const uint64_t pend = __ballot(pending);
+    if (pend == 0) return false;
+    const int np = __popcll(pend);
+    return np >= bnd_min || pend == __ballot(alive);
+}
+
 } // namespace
 
 /* ----------------------------------------------------------------- first pass */
 extern "C" __global__ void __launch_bounds__(256) mcml_first_pass(FirstPassArgs a)
 {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const DevSet* sets = a.sets;
+    if (a.lds_sets) { /* stage the (small) set table in LDS; reads go through one pointer */
+        const uint4* src = reinterpret_cast<const uint4*>(a.sets);
+        uint4* dst = reinterpret_cast<uint4*>(lds_raw);
+        const int n16 = (int)(sizeof(DevSet) / 16) * a.nsets;
+        for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+        sets = reinterpret_cast<const DevSet*>(lds_raw);
+    }
     Photon p;
-    bool alive = false;
-    const DevSet* S = a.sets;
-    uint32_t set = 0;
+    bool alive = false, pending = false;
+    int pend_layer = 0;
+    const DevSet* S = sets;
     uint64_t key = 0;
-    /* wavefront-local ticket range [q_next, q_end); uniform across the wavefront */
-    uint64_t q_next = 0, q_end = 0;
-    bool q_dry = false;
-    uint64_t c_photons = 0, c_steps = 0;
-    uint32_t c_bnd = 0, c_int = 0;
-    uint64_t c_bnd64 = 0, c_int64 = 0;
-    uint64_t rl_idx = ~0ull, rl_w = 0;
+    WaveQueue q = {0, 0, false};
+    uint64_t c_photons = 0, c_steps = 0, c_bnd = 0, c_int = 0;
+    uint32_t n_int = 0;
+    GridLane gl = {nullptr, nullptr, ~0ull, 0};
     const int refill_min = a.refill_min < 1 ? 1 : a.refill_min;
+    const int bnd_min = a.bnd_min < 1 ? 1 : a.bnd_min;
 
     for (;;) {
         const uint64_t dead = __ballot(!alive);
         const int ndead = __popcll(dead);
-        if (!q_dry && (ndead >= refill_min || ndead == 64)) {
-            const uint32_t rank = rank_in(dead);
-            const uint64_t avail = q_end - q_next;
-            uint64_t t = ~0ull;
-            if (!alive && rank < avail) t = q_next + rank;
-            if ((uint64_t)ndead > avail) {
-                unsigned long long base = 0;
-                if (lane_id() == (uint32_t)(__ffsll((long long)__ballot(1)) - 1))
-                    base = atomicAdd(a.queue, (unsigned long long)a.chunk);
-                base = bcast_first(base);
-                uint64_t cend = base + (uint64_t)a.chunk;
-                if (cend > a.total) cend = a.total;
-                if (base >= a.total) {
-                    q_dry = true;
-                    cend = base;
-                }
-                if (!alive && rank >= avail) {
-                    const uint64_t tt = base + (rank - avail);
-                    if (tt < cend) t = tt;
-                }
-                const uint64_t used = base + ((uint64_t)ndead - avail);
-                q_next = used < cend ? used : cend;
-                q_end = cend;
-            } else {
-                q_next += (uint64_t)ndead;
-            }
+        if (!q.dry && (ndead >= refill_min || ndead == 64)) {
+            const uint64_t t = take_ids(q, dead, !alive, a.queue, (uint64_t)a.chunk, a.total);
             if (t != ~0ull) {
-                set = a.nsets > 1 ? set_of_ticket(a.set_begin, a.nsets, t) : 0u;
-                S = a.sets + set;
+                const uint32_t set = a.nsets > 1 ? set_of_ticket(a.set_begin, a.nsets, t) : 0u;
+                S = sets + set;
                 const uint64_t k = t - a.set_begin[set];
                 key = ((uint64_t)set << 40) | k;
                 launch(p, S, a.seed, set, k, a.mult);
@@ -404,36 +467,47 @@ extern "C" __global__ void __launch_bounds__(256) mcml_first_pass(FirstPassArgs 
             }
         }
         if (__ballot(alive) == 0) break;
-        if (alive) {
-            uint32_t mask = 0;
-            const int st = hop_drop_spin<false>(p, S, 0, &mask, nullptr, 0, nullptr, nullptr, nullptr, rl_idx, rl_w,
-                                                c_bnd, c_int);
-            if (st != ST_ALIVE) {
-                alive = false;
-                c_steps += (uint64_t)p.steps;
-                c_bnd64 += c_bnd;
-                c_int64 += c_int;
-                c_bnd = 0;
-                c_int = 0;
-                if (st == ST_DETECTED) {
-                    const unsigned long long base = atomicAdd(a.n_records, (unsigned long long)__popc(mask));
-                    unsigned long long idx = base;
-                    for (int i = 1; i <= S->num_sds; i++) {
-                        if (!(mask & (1u << i))) continue;
-                        if (idx < a.cap) {
-                            mcml_record r;
-                            r.key = key;
-                            r.sds = (uint32_t)i;
-                            r.weight = credited(p, S, i);
-                            a.records[idx] = r;
-                        }
-                        idx++;
+        int st = ST_ALIVE;
+        uint32_t mask = 0;
+        if (alive && !pending) {
+            float s;
+            int nl;
+            if (hop<false>(p, s, nl, nullptr, 0)) {
+                pending = true;
+                pend_layer = nl;
+            } else {
+                st = interact<false>(p, s, gl, nullptr, n_int);
+            }
+        }
+        if (resolve_now(pending, alive, bnd_min) && pending) {
+            pending = false;
+            c_bnd++;
+            st = cross(p, pend_layer, S, 0, mask);
+        }
+        if (st != ST_ALIVE) {
+            alive = false;
+            c_steps += (uint64_t)p.steps;
+            c_int += n_int;
+            n_int = 0;
+            if (st == ST_DETECTED) {
+                const unsigned long long base = atomicAdd(a.n_records, (unsigned long long)__popc(mask));
+                unsigned long long idx = base;
+                const int ns = S->num_sds;
+                for (int i = 1; i <= ns; i++) {
+                    if (!(mask & (1u << i))) continue;
+                    if (idx < a.cap) {
+                        mcml_record r;
+                        r.key = key;
+                        r.sds = (uint32_t)i;
+                        r.weight = credited(p, S, i);
+                        a.records[idx] = r;
                     }
+                    idx++;
                 }
             }
         }
     }
-    const uint64_t s0 = wave_sum(c_photons), s1 = wave_sum(c_steps), s2 = wave_sum(c_bnd64), s3 = wave_sum(c_int64);
+    const uint64_t s0 = wave_sum(c_photons), s1 = wave_sum(c_steps), s2 = wave_sum(c_bnd), s3 = wave_sum(c_int);
     if (lane_id() == 0) {
         atomicAdd(&a.counters[CNT_PHOTONS], (unsigned long long)s0);
         atomicAdd(&a.counters[CNT_STEPS], (unsigned long long)s1);
@@ -449,65 +523,59 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
 {
     /* per-lane pathlength accumulators, layer-major so lanes never share a bank word */
     __shared__ float pl_lds[MCML_MAX_LAYERS * REPLAY_BLOCK];
-    const int tid = threadIdx.x;
-    float* pl_lane = pl_lds + tid;
+    float* pl_lane = pl_lds + threadIdx.x;
     const int L = a.num_layers;
-    uint64_t q_next = 0, q_end = 0;
-    bool q_dry = false;
-    uint32_t dummy_b = 0, dummy_i = 0;
+    const int bnd_min = a.bnd_min < 1 ? 1 : a.bnd_min;
+    WaveQueue q = {0, 0, false};
+    Photon p;
+    bool alive = false, pending = false;
+    int pend_layer = 0;
+    uint64_t job = 0;
+    ReplayJob J = {0, 0, 0};
+    const DevSet* S = a.sets;
+    GridLane gl = {nullptr, nullptr, ~0ull, 0};
+    uint32_t n_int = 0;
 
     for (;;) {
-        /* every lane takes a fresh job each round: jobs are independent and short */
-        const uint64_t all = __ballot(1);
-        const int nl = __popcll(all);
-        const uint32_t rank = rank_in(all);
-        uint64_t job = ~0ull;
-        if (!q_dry) {
-            const uint64_t avail = q_end - q_next;
-            if (rank < avail) job = q_next + rank;
-            if ((uint64_t)nl > avail) {
-                unsigned long long base = 0;
-                if (rank == 0) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
-                base = bcast_first(base);
-                uint64_t cend = base + (uint64_t)a.chunk;
-                if (cend > a.njobs) cend = a.njobs;
-                if (base >= a.njobs) {
-                    q_dry = true;
-                    cend = base;
+        const uint64_t idle = __ballot(!alive);
+        if (!q.dry && idle != 0) {
+            const uint64_t t = take_ids(q, idle, !alive, a.queue, (uint64_t)a.chunk, a.njobs);
+            if (t != ~0ull) {
+                job = t;
+                J = a.jobs[job];
+                S = a.sets + J.set;
+                launch(p, S, a.seed, J.set, J.k, a.mult);
+                for (int l = 0; l < L; l++) pl_lane[l * REPLAY_BLOCK] = 0.0f;
+                gl.rl_idx = ~0ull;
+                gl.rl_w = 0;
+                if (a.A_rz != nullptr) {
+                    const uint64_t slab = (uint64_t)J.set * (uint64_t)a.num_sds + (uint64_t)(J.sds - 1);
+                    gl.a_rz = a.A_rz + slab * (uint64_t)a.nz * (uint64_t)a.nr;
+                    gl.a0_z = a.A0_z + slab * (uint64_t)a.nz;
                 }
-                if (rank >= avail) {
-                    const uint64_t jj = base + (rank - avail);
-                    if (jj < cend) job = jj;
-                }
-                const uint64_t used = base + ((uint64_t)nl - avail);
-                q_next = used < cend ? used : cend;
-                q_end = cend;
-            } else {
-                q_next += (uint64_t)nl;
+                alive = true;
             }
         }
-        if (__ballot(job != ~0ull) == 0) break;
-        if (job != ~0ull) {
-            const ReplayJob J = a.jobs[job];
-            const DevSet* S = a.sets + J.set;
-            Photon p;
-            launch(p, S, a.seed, J.set, J.k, a.mult);
-            for (int l = 0; l < L; l++) pl_lane[l * REPLAY_BLOCK] = 0.0f;
-            unsigned long long* a_rz = nullptr;
-            unsigned long long* a0_z = nullptr;
-            if (a.A_rz != nullptr) {
-                const uint64_t slab = (uint64_t)J.set * (uint64_t)a.num_sds + (uint64_t)(J.sds - 1);
-                a_rz = a.A_rz + slab * (uint64_t)a.nz * (uint64_t)a.nr;
-                a0_z = a.A0_z + slab * (uint64_t)a.nz;
+        if (__ballot(alive) == 0) break;
+        int st = ST_ALIVE;
+        uint32_t mask = 0;
+        if (alive && !pending) {
+            float s;
+            int nl;
+            if (hop<true>(p, s, nl, pl_lane, REPLAY_BLOCK)) {
+                pending = true;
+                pend_layer = nl;
+            } else {
+                st = interact<true>(p, s, gl, &a, n_int);
             }
-            uint64_t rl_idx = ~0ull, rl_w = 0;
-            uint32_t mask = 0;
-            int st;
-            do {
-                st = hop_drop_spin<true>(p, S, (int)J.sds, &mask, pl_lane, REPLAY_BLOCK, a_rz, a0_z, &a, rl_idx,
-                                         rl_w, dummy_b, dummy_i);
-            } while (st == ST_ALIVE);
-            if (a_rz != nullptr && rl_w != 0) atomicAdd(&a_rz[rl_idx], (unsigned long long)rl_w);
+        }
+        if (resolve_now(pending, alive, bnd_min) && pending) {
+            pending = false;
+            st = cross(p, pend_layer, S, (int)J.sds, mask);
+        }
+        if (st != ST_ALIVE) {
+            alive = false;
+            if (gl.a_rz != nullptr && gl.rl_w != 0) atomicAdd(&gl.a_rz[gl.rl_idx], (unsigned long long)gl.rl_w);
             a.weight[job] = (st == ST_DETECTED) ? credited(p, S, (int)J.sds) : 0.0f;
             a.steps[job] = (float)p.steps;
             for (int l = 0; l < L; l++) a.pl[(uint64_t)l * a.njobs + job] = pl_lane[l * REPLAY_BLOCK];
@@ -538,7 +606,8 @@ extern "C" __global__ void __launch_bounds__(256) mcml_dump(DumpArgs a)
 
 hipError_t mcml_launch_first_pass(const FirstPassArgs& a, int grid, int block, hipStream_t st)
 {
-    hipLaunchKernelGGL(mcml_first_pass, dim3(grid), dim3(block), 0, st, a);
+    const size_t lds = a.lds_sets ? sizeof(DevSet) * (size_t)a.nsets : 0;
+    hipLaunchKernelGGL(mcml_first_pass, dim3(grid), dim3(block), lds, st, a);
     return hipGetLastError();
 }
 
@@ -554,9 +623,9 @@ hipError_t mcml_launch_dump(const DumpArgs& a, int grid, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t mcml_occupancy_first_pass(int block, int* blocks_per_cu)
+hipError_t mcml_occupancy_first_pass(int block, size_t lds_bytes, int* blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_first_pass, block, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_first_pass, block, lds_bytes);
 }
 
 hipError_t mcml_occupancy_replay(int* blocks_per_cu)

@@ -154,11 +154,15 @@ def test_seed_changes_streams(ctx):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("block,chunk,refill", [(64, 64, 1), (128, 128, 8), (256, 1024, 32), (192, 64, 64)])
-def test_launch_shape_does_not_change_results(ctx, block, chunk, refill):
+@pytest.mark.parametrize("block,chunk,refill,bnd", [(64, 64, 1, 1), (128, 128, 8, 4), (256, 1024, 32, 64),
+                                                    (192, 64, 64, 16)])
+def test_launch_shape_does_not_change_results(ctx, block, chunk, refill, bnd):
     sim = F.ring_busy(30000)
-    ref = ctx.run(sim, make_opts(), seed=SEED)
-    got = ctx.run(sim, make_opts(block_size=block, chunk=chunk, refill_min=refill), seed=SEED)
-    assert got.block_size == block and got.chunk == chunk
+    opts = make_opts(replay=True, pathlength=True)
+    ref = ctx.run(sim, opts, seed=SEED)
+    got = ctx.run(sim, make_opts(replay=True, pathlength=True, block_size=block, chunk=chunk, refill_min=refill,
+                                 boundary_min=bnd), seed=SEED)
+    assert got.block_size == block and got.chunk == chunk and got.boundary_min == bnd
     assert np.array_equal(got.records, ref.records)
     assert got.steps == ref.steps
+    assert np.array_equal(got.pathlength, ref.pathlength)
