@@ -68,6 +68,8 @@ def main():
     import numpy as np
     import mcml_gpu_amd as M
 
+    if args.config == "cfg3":
+        return bench_batch(args)
     sim = M.load_config(args.config)
     sim.number_photons = int(args.photons)
     opts = M.make_opts(replay=True, pathlength=True, binary=True, block_size=args.block,
@@ -151,6 +153,50 @@ def main():
             "value": sample / dt, "unit": "photons/s", "cores": cores, "kind": "port",
             "sample": "%d photons of the same config, first pass (oracle, OpenMP), %.1f s" % (sample, dt),
         }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def bench_batch(args):
+    """cfg3: 512 three-layer sets with log-uniform budgets (about 1.1e9 photons) in one
+    persistent launch; launch shape picked by mcml_autotune."""
+    import numpy as np
+    import mcml_gpu_amd as M
+    from mcml_gpu_amd import batch
+
+    sims, params, budgets = batch.cfg3_sets(512, seed=1)
+    total = int(budgets.sum())
+    ctx = M.Context(0)
+    tuned = ctx.autotune(sims, trial_photons=2e7)
+    opts = M.make_opts(block_size=tuned.block_size, chunk=tuned.chunk, refill_min=tuned.refill_min,
+                       use_graph=True, skip_records=True)
+    for _ in range(args.warmup):
+        ctx.run_batch(sims, opts, seed=args.seed)
+    kern, walls_fp = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        res = ctx.run_batch(sims, opts, seed=args.seed + i)
+        kern.append(res.ms_first_pass_kernel)
+        walls_fp.append(res.ms_first_pass_wall)
+    elapsed = time.perf_counter() - t0
+    k_ms = float(np.mean(kern))
+    flop = (FLOP_INTERACTION * res.steps_interaction + FLOP_BOUNDARY * res.steps_boundary)
+    achieved = flop / (k_ms * 1e-3) / 1e12
+    line = {
+        "metric": "photons/sec/GPU (plain MC, batched)", "value": total * args.steps / elapsed,
+        "unit": "photons/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "cfg3: 512 three-layer sets, budgets log-uniform 1e5-1e7, one persistent launch",
+                   "photons": total, "sets": len(sims), "block": res.block_size, "chunk": res.chunk,
+                   "refill_min": res.refill_min, "graph": True},
+        "kernel_photons_per_s": total / (k_ms * 1e-3), "first_pass_kernel_ms": k_ms,
+        "first_pass_wall_ms": float(np.mean(walls_fp)), "launches": res.first_pass_launches,
+        "detected": int(res.detected.sum()), "steps_per_photon": res.steps / total,
+        "roofline": {"bound": "valu", "kernel": "mcml_first_pass", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": None},
+    }
     print(json.dumps(line), flush=True)
     ctx.close()
 

@@ -135,8 +135,12 @@ typedef struct {
     int32_t refill_min;      /* refill when at least this many lanes are idle; 0 = tuned */
     int32_t boundary_min;    /* resolve boundary hits when at least this many lanes wait on
                                 one; 0 = tuned (results never depend on the launch knobs) */
-    int32_t reference_shape; /* 1 = emulate the reference launch shape (32-lane groups,
-                                one photon per lane, 50000-step relaunch with host sync) */
+    int32_t reference_shape; /* reserved, must be 0 */
+    int32_t use_graph;       /* 1 = run the first pass through a cached HIP graph (reset
+                                kernel + transport kernel + counter read-back), updating
+                                only its kernel arguments between calls */
+    int32_t skip_records;    /* 1 = do not return the per-photon detection records (sums,
+                                counts and pathlength rows are still returned) */
 } mcml_opts;
 
 /* First-pass detection record: photon `key` = (set << 40) | photon index, detected by
@@ -157,7 +161,7 @@ typedef struct {
     double* reflectance;        /* [nsets*S] sum of detected weight, W-units */
     uint64_t* detected;         /* [nsets*S] detected photon count */
     uint64_t n_records;
-    mcml_record* records;       /* sorted by (key, sds) */
+    mcml_record* records;       /* sorted by (set, sds, photon index); NULL with skip_records */
     /* replay (opts.replay) */
     int32_t replayed;
     double* replay_reflectance; /* [nsets*S] */
@@ -208,6 +212,13 @@ int mcml_run_batch(mcml_ctx* ctx, const mcml_sim* sims, int nsets, const mcml_op
                    uint64_t seed, mcml_result* out);
 
 void mcml_result_free(mcml_result* res);
+
+/* Pick the launch shape (block_size, chunk, refill_min) for these parameter sets: times the
+ * first pass of a reduced copy of the batch (about `trial_photons` photons in total) for
+ * each candidate shape and writes the fastest into *opts (other fields untouched).  No
+ * counterpart in the reference, which sizes its grid from compute-capability tables
+ * (io.cpp:471-493). */
+int mcml_autotune(mcml_ctx* ctx, const mcml_sim* sims, int nsets, uint64_t trial_photons, mcml_opts* opts);
 
 /* Average pathlength per SDS and layer, ln(sum w / sum w e^{-dmua L}) / dmua with
  * dmua = 0.001 mua (transport.cu:274-297).  out: [S*L] (SDS-major). */

@@ -57,6 +57,7 @@ def load_library():
     L.mcml_write_outputs.argtypes = [P(abi.Sim), P(abi.Opts), P(abi.Result), C.c_int, C.c_char_p,
                                      C.c_char_p, P(abi.SummaryInfo)]
     L.mcml_main.argtypes = [C.c_int, P(C.c_char_p)]
+    L.mcml_autotune.argtypes = [C.c_void_p, P(abi.Sim), C.c_int, C.c_uint64, P(abi.Opts)]
     _lib = L
     return L
 
@@ -90,9 +91,11 @@ def load_config(name):
 
 
 def make_opts(replay=False, absorbance=False, pathlength=False, avg_pathlength=False, binary=False,
-              block_size=0, blocks_per_cu=0, chunk=0, refill_min=0, boundary_min=0, reference_shape=0):
+              block_size=0, blocks_per_cu=0, chunk=0, refill_min=0, boundary_min=0, use_graph=False,
+              skip_records=False):
     return abi.Opts(int(replay), int(absorbance), int(pathlength), int(avg_pathlength), int(binary),
-                    block_size, blocks_per_cu, chunk, refill_min, boundary_min, reference_shape)
+                    block_size, blocks_per_cu, chunk, refill_min, boundary_min, 0, int(use_graph),
+                    int(skip_records))
 
 
 class RunResult:
@@ -103,7 +106,7 @@ class RunResult:
         g = r.nsets * r.num_sds
         self.reflectance = np.ctypeslib.as_array(r.reflectance, (g,)).copy().reshape(r.nsets, r.num_sds)
         self.detected = np.ctypeslib.as_array(r.detected, (g,)).copy().reshape(r.nsets, r.num_sds)
-        n = r.n_records
+        n = r.n_records if r.records else 0
         if n:
             raw = np.ctypeslib.as_array(C.cast(r.records, C.POINTER(C.c_uint8)), (n * 16,)).copy()
             self.records = raw.view(np.dtype(abi.RECORD_DTYPE))
@@ -180,6 +183,15 @@ class Context:
             return RunResult(r)
         finally:
             self.lib.mcml_result_free(C.byref(r))
+
+    def autotune(self, sims, trial_photons=10_000_000, opts=None):
+        """Launch shape picked by mcml_autotune for these sets (returns a filled Opts)."""
+        if isinstance(sims, abi.Sim):
+            sims = [sims]
+        arr = (abi.Sim * len(sims))(*sims)
+        o = opts if opts is not None else make_opts()
+        self._check(self.lib.mcml_autotune(self.h, arr, len(sims), int(trial_photons), C.byref(o)))
+        return o
 
     def close(self):
         if self.h:

@@ -93,7 +93,8 @@ class Derived(C.Structure):
 class Opts(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "replay", "absorbance", "pathlength", "avg_pathlength", "binary",
-        "block_size", "blocks_per_cu", "chunk", "refill_min", "boundary_min", "reference_shape")]
+        "block_size", "blocks_per_cu", "chunk", "refill_min", "boundary_min", "reference_shape",
+        "use_graph", "skip_records")]
 
 
 class Record(C.Structure):

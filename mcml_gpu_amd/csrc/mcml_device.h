@@ -48,6 +48,14 @@ struct alignas(16) DevSet {
 /* Work counters accumulated by the first-pass kernel (one atomic per lane at exit). */
 enum { CNT_PHOTONS = 0, CNT_STEPS = 1, CNT_BOUNDARY = 2, CNT_INTERACTION = 3, CNT_N = 4 };
 
+/* Detection-record sort key: set (19 bits) | SDS (5 bits) | photon index (40 bits), so
+ * that ascending keys group records by (set, SDS) in photon order -- the order of the
+ * pathlength_SDS_# files. */
+#define MCML_KEY_SET_SHIFT 45
+#define MCML_KEY_SDS_SHIFT 40
+#define MCML_KEY_K_MASK ((1ull << 40) - 1ull)
+#define MCML_MAX_SETS (1 << 19)
+
 struct FirstPassArgs {
     const DevSet* sets;
     const uint64_t* set_begin; /* [nsets+1] ticket prefix: set j owns [set_begin[j], set_begin[j+1]) */
@@ -56,27 +64,23 @@ struct FirstPassArgs {
     int32_t refill_min;        /* refill once this many lanes of the wavefront are idle */
     int32_t bnd_min;           /* resolve boundary hits once this many lanes wait on one */
     int32_t lds_sets;          /* 1: stage the set table in LDS (dynamic shared memory) */
-    int32_t pad;
+    int32_t num_layers;        /* L (pathlength recording) */
     uint64_t total;            /* total tickets */
     uint64_t seed;
     const uint32_t* mult;      /* MWC multiplier table */
     unsigned long long* queue;     /* global ticket counter */
-    mcml_record* records;
+    uint64_t* rec_key;             /* [cap] sort keys of detection records */
+    float* rec_row;                /* [cap * row_stride]: weight, or [weight, PL_1..PL_L, steps] */
+    int32_t row_stride;            /* 1, or L + 2 when the kernel records pathlengths */
+    int32_t pad;
     unsigned long long* n_records; /* records appended (may exceed cap) */
     uint64_t cap;
     unsigned long long* counters;  /* [CNT_N] */
 };
 
-/* One replay job: photon k of set j, replayed for detector sds (1-based). */
-struct ReplayJob {
-    uint64_t k;
-    uint32_t set;
-    uint32_t sds;
-};
-
 struct ReplayArgs {
     const DevSet* sets;
-    const ReplayJob* jobs;
+    const uint64_t* keys;      /* [njobs] sorted record keys: each is one replay job */
     uint64_t njobs;
     uint64_t seed;
     const uint32_t* mult;
@@ -85,10 +89,7 @@ struct ReplayArgs {
     int32_t num_layers;
     int32_t bnd_min;
     int32_t pad2;
-    /* outputs, structure of arrays over jobs */
-    float* weight;             /* [njobs] detected weight (0 if not re-detected) */
-    float* steps;              /* [njobs] step count as float */
-    float* pl;                 /* [num_layers * njobs], layer-major */
+    float* rows;               /* [njobs * (L+2)]: [weight, PL_1..PL_L, steps] */
     /* absorbance grids (NULL = off): per (set, sds) slab of nz*nr / nz cells */
     unsigned long long* A_rz;
     unsigned long long* A0_z;
@@ -96,14 +97,17 @@ struct ReplayArgs {
     float dr, dz;
 };
 
-struct DumpArgs {
-    const float* weight;
-    const float* steps;
-    const float* pl;
-    uint64_t njobs;
-    int32_t num_layers;
-    int32_t pad;
-    float* out;                /* [njobs * (num_layers+2)], records as written to disk */
+/* Records in sorted order: gather rows, build the public records, reduce per (set, SDS). */
+struct GatherArgs {
+    const uint64_t* sorted_keys;
+    const uint32_t* order;     /* source index of each sorted record (NULL = identity) */
+    const float* rows_in;
+    int32_t stride_in;         /* floats per source row */
+    int32_t stride_out;        /* floats per gathered row (0 = do not gather rows) */
+    uint64_t n;
+    float* rows_out;
+    float* weight_out;         /* [n] sorted weights */
+    mcml_record* records_out;  /* [n] public records (NULL = skip) */
 };
 
 #endif /* MCML_DEVICE_H */

@@ -4,10 +4,30 @@
 #include <hip/hip_runtime.h>
 #include "mcml_device.h"
 
+/* Cached HIP graph of one first pass (see mcml_graph_first_pass). */
+struct FirstPassGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t reset = nullptr, kernel = nullptr, readback = nullptr;
+    int grid = 0, block = 0, nctrl = 0;
+    size_t lds = 0;
+    unsigned long long* ctrl = nullptr;
+    void* host_ctrl = nullptr;
+    void* func = nullptr;
+};
+
+hipError_t mcml_launch_reset(unsigned long long* ctrl, int n, hipStream_t st);
+hipError_t mcml_graph_first_pass(FirstPassGraph& G, const FirstPassArgs& a, int grid, int block,
+                                 unsigned long long* ctrl, int nctrl, void* host_ctrl, hipStream_t st);
+void mcml_graph_release(FirstPassGraph& G);
 hipError_t mcml_launch_first_pass(const FirstPassArgs& a, int grid, int block, hipStream_t st);
 hipError_t mcml_launch_replay(const ReplayArgs& a, int grid, hipStream_t st);
-hipError_t mcml_launch_dump(const DumpArgs& a, int grid, hipStream_t st);
-hipError_t mcml_occupancy_first_pass(int block, size_t lds_bytes, int* blocks_per_cu);
+hipError_t mcml_launch_dump(const GatherArgs& a, int grid, hipStream_t st);
+hipError_t mcml_launch_groups(const uint64_t* keys, uint64_t n, int nsets, int nsds, uint64_t* offsets,
+                              const float* w, double* sums, hipStream_t st);
+hipError_t mcml_occupancy_first_pass(const FirstPassArgs& a, int block, int* blocks_per_cu);
+hipError_t mcml_sort_keys(void* temp, size_t& temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
+                          uint32_t* idx_out, uint64_t n, hipStream_t st);
 hipError_t mcml_occupancy_replay(int* blocks_per_cu);
 int mcml_replay_block();
 

@@ -35,6 +35,9 @@ constexpr int kDefaultChunk = 128;
 constexpr int kDefaultRefill = 2;
 constexpr int kDefaultBoundary = 1;
 constexpr size_t kLdsTableMax = 16384;
+constexpr int kCtrlWords = 8;
+constexpr int kCtrlFirstPass = 6; /* words the first pass resets and reads back */
+constexpr int kCtrlReplayQueue = 6;
 
 struct DevBuf {
     void* p = nullptr;
@@ -51,7 +54,17 @@ struct mcml_ctx {
     char name[256] = {0};
     char err[1024] = {0};
     uint32_t* d_mult = nullptr;
-    DevBuf sets, set_begin, queue, records, nrec, counters, jobs, rw, rs, rpl, dump, arz, a0z;
+    /* control block: [0] first-pass ticket counter, [1] records appended, [2..5] work
+     * counters, [6] replay job counter; mirrored into pinned host memory */
+    unsigned long long* ctrl = nullptr;
+    unsigned long long* h_ctrl = nullptr;
+    FirstPassGraph graph;
+    DevBuf sets, set_begin;
+    DevBuf rec_key, rec_row;          /* detection records as appended by the first pass */
+    DevBuf skey, sidx, stemp;         /* sorted keys, their source indices, sort scratch */
+    DevBuf rows, weights, precs;      /* sorted rows (file image), weights, public records */
+    DevBuf offsets, sums;             /* per (set, SDS) group: start offsets, FP64 sums */
+    DevBuf rrows, arz, a0z;           /* replay rows and absorbance grids */
 };
 
 namespace {
@@ -199,10 +212,6 @@ void fill_devset(const mcml_derived& d, DevSet& S)
     }
 }
 
-bool rec_less(const mcml_record& a, const mcml_record& b)
-{
-    return a.key < b.key || (a.key == b.key && a.sds < b.sds);
-}
 
 } // namespace
 
@@ -305,7 +314,10 @@ extern "C" mcml_ctx* mcml_create(int device, int* err)
     }
     c->cu_count = prop.multiProcessorCount;
     snprintf(c->name, sizeof(c->name), "%s", prop.name[0] ? prop.name : prop.gcnArchName);
-    if (hipMalloc((void**)&c->d_mult, sizeof(uint32_t) * MCML_MWC_TABLE_SIZE) != hipSuccess ||
+    if (hipMalloc((void**)&c->ctrl, sizeof(unsigned long long) * kCtrlWords) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_ctrl, sizeof(unsigned long long) * kCtrlWords, hipHostMallocDefault) !=
+            hipSuccess ||
+        hipMalloc((void**)&c->d_mult, sizeof(uint32_t) * MCML_MWC_TABLE_SIZE) != hipSuccess ||
         hipMemcpy(c->d_mult, mcml_mwc_multipliers, sizeof(uint32_t) * MCML_MWC_TABLE_SIZE,
                   hipMemcpyHostToDevice) != hipSuccess) {
         *err = MCML_ERR_DEVICE;
@@ -321,10 +333,14 @@ extern "C" void mcml_destroy(mcml_ctx* c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevBuf* bufs[] = {&c->sets, &c->set_begin, &c->queue, &c->records, &c->nrec, &c->counters, &c->jobs,
-                      &c->rw, &c->rs, &c->rpl, &c->dump, &c->arz, &c->a0z};
+    DevBuf* bufs[] = {&c->sets,  &c->set_begin, &c->rec_key, &c->rec_row, &c->skey,  &c->sidx, &c->stemp,
+                      &c->rows,  &c->weights,   &c->precs,   &c->offsets, &c->sums,  &c->rrows, &c->arz,
+                      &c->a0z};
     for (DevBuf* b : bufs) release(*b);
+    mcml_graph_release(c->graph);
     if (c->d_mult) (void)hipFree(c->d_mult);
+    if (c->ctrl) (void)hipFree(c->ctrl);
+    if (c->h_ctrl) (void)hipHostFree(c->h_ctrl);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -357,8 +373,8 @@ int validate_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, std::vector<mcm
         set_err(c, "nsets must be >= 1");
         return MCML_ERR_ARG;
     }
-    if ((uint64_t)nsets >= (1ull << 23)) {
-        set_err(c, "too many parameter sets (%d)", nsets);
+    if (nsets >= MCML_MAX_SETS) {
+        set_err(c, "too many parameter sets (%d, at most %d)", nsets, MCML_MAX_SETS - 1);
         return MCML_ERR_LIMIT;
     }
     der.resize((size_t)nsets);
@@ -380,10 +396,12 @@ int validate_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, std::vector<mcm
     return MCML_OK;
 }
 
-int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opts& o, uint64_t seed,
-               mcml_result* out, std::vector<mcml_record>& recs)
+/* First pass: leaves n_rec records in c->rec_key / c->rec_row (row_stride floats each). */
+int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opts& o, uint64_t seed, int row_stride,
+               mcml_result* out, uint64_t& n_rec)
 {
     const int nsets = (int)der.size();
+    const int S = der[0].num_sds;
     std::vector<DevSet> hs((size_t)nsets);
     std::vector<uint64_t> begin((size_t)nsets + 1, 0);
     for (int j = 0; j < nsets; j++) {
@@ -394,10 +412,7 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
     const uint64_t total = begin[(size_t)nsets];
     int rc;
     if ((rc = ensure(c, c->sets, sizeof(DevSet) * (size_t)nsets)) ||
-        (rc = ensure(c, c->set_begin, sizeof(uint64_t) * ((size_t)nsets + 1))) ||
-        (rc = ensure(c, c->queue, sizeof(unsigned long long))) ||
-        (rc = ensure(c, c->nrec, sizeof(unsigned long long))) ||
-        (rc = ensure(c, c->counters, sizeof(unsigned long long) * CNT_N)))
+        (rc = ensure(c, c->set_begin, sizeof(uint64_t) * ((size_t)nsets + 1))))
         return rc;
     HIP_TRY(c, hipMemcpyAsync(c->sets.p, hs.data(), sizeof(DevSet) * (size_t)nsets, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->set_begin.p, begin.data(), sizeof(uint64_t) * ((size_t)nsets + 1),
@@ -408,65 +423,68 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
         set_err(c, "block_size must be a multiple of 64 and at most 256");
         return MCML_ERR_ARG;
     }
+    FirstPassArgs a;
+    memset(&a, 0, sizeof(a));
+    a.sets = as<DevSet>(c->sets);
+    a.set_begin = as<uint64_t>(c->set_begin);
+    a.nsets = nsets;
+    a.num_layers = der[0].num_layers;
+    a.row_stride = row_stride;
     /* a small set table is staged in LDS by every workgroup; a large batch reads it from
      * global memory (it is only read at launch, layer changes and boundary hits) */
-    const size_t table_bytes = sizeof(DevSet) * (size_t)nsets;
-    const int lds_sets = table_bytes <= kLdsTableMax ? 1 : 0;
+    a.lds_sets = sizeof(DevSet) * (size_t)nsets <= kLdsTableMax ? 1 : 0;
     int occ = 0;
-    HIP_TRY(c, mcml_occupancy_first_pass(block, lds_sets ? table_bytes : 0, &occ));
+    HIP_TRY(c, mcml_occupancy_first_pass(a, block, &occ));
     if (occ < 1) occ = 1;
     const int bpc = o.blocks_per_cu > 0 ? std::min(o.blocks_per_cu, occ) : occ;
     const int grid = c->cu_count * bpc;
-    const int chunk = o.chunk > 0 ? std::max(o.chunk, 64) : kDefaultChunk;
-    const int refill_min = o.refill_min > 0 ? std::min(o.refill_min, 64) : kDefaultRefill;
-    const int bnd_min = o.boundary_min > 0 ? std::min(o.boundary_min, 64) : kDefaultBoundary;
+    a.chunk = o.chunk > 0 ? std::max(o.chunk, 64) : kDefaultChunk;
+    a.refill_min = o.refill_min > 0 ? std::min(o.refill_min, 64) : kDefaultRefill;
+    a.bnd_min = o.boundary_min > 0 ? std::min(o.boundary_min, 64) : kDefaultBoundary;
+    a.total = total;
+    a.seed = seed;
+    a.mult = c->d_mult;
+    a.queue = c->ctrl + 0;
+    a.n_records = c->ctrl + 1;
+    a.counters = c->ctrl + 2;
 
-    /* record capacity: detection is rare for small fibers; grow and rerun on overflow
-     * (the result does not depend on the capacity, each photon's stream is fixed) */
-    uint64_t cap = std::max<uint64_t>(1u << 16, std::min<uint64_t>(total / 16 + 1024, 1ull << 26));
-    uint64_t n_rec = 0;
-    unsigned long long cnt[CNT_N];
+    /* record capacity: a quarter of the photons (or every photon times every detector when
+     * that is smaller); on overflow the pass reruns with the exact count -- the result does
+     * not depend on the capacity, each photon's stream is fixed */
+    uint64_t cap = std::min<uint64_t>(total * (uint64_t)S, std::max<uint64_t>(1ull << 20, total / 4 + 1024));
+    unsigned long long cnt[CNT_N] = {0, 0, 0, 0};
     double ms_kernel = 0;
     int launches = 0;
     for (;;) {
-        if ((rc = ensure(c, c->records, sizeof(mcml_record) * (size_t)cap))) return rc;
-        HIP_TRY(c, hipMemsetAsync(c->queue.p, 0, sizeof(unsigned long long), c->stream));
-        HIP_TRY(c, hipMemsetAsync(c->nrec.p, 0, sizeof(unsigned long long), c->stream));
-        HIP_TRY(c, hipMemsetAsync(c->counters.p, 0, sizeof(unsigned long long) * CNT_N, c->stream));
-        FirstPassArgs a;
-        memset(&a, 0, sizeof(a));
-        a.sets = as<DevSet>(c->sets);
-        a.set_begin = as<uint64_t>(c->set_begin);
-        a.nsets = nsets;
-        a.chunk = chunk;
-        a.refill_min = refill_min;
-        a.bnd_min = bnd_min;
-        a.lds_sets = lds_sets;
-        a.total = total;
-        a.seed = seed;
-        a.mult = c->d_mult;
-        a.queue = as<unsigned long long>(c->queue);
-        a.records = as<mcml_record>(c->records);
-        a.n_records = as<unsigned long long>(c->nrec);
+        if ((rc = ensure(c, c->rec_key, sizeof(uint64_t) * (size_t)cap)) ||
+            (rc = ensure(c, c->rec_row, sizeof(float) * (size_t)cap * (size_t)row_stride)))
+            return rc;
+        a.rec_key = as<uint64_t>(c->rec_key);
+        a.rec_row = as<float>(c->rec_row);
         a.cap = cap;
-        a.counters = as<unsigned long long>(c->counters);
         HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-        HIP_TRY(c, mcml_launch_first_pass(a, grid, block, c->stream));
-        HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+        if (o.use_graph) {
+            HIP_TRY(c, mcml_graph_first_pass(c->graph, a, grid, block, c->ctrl, kCtrlFirstPass, c->h_ctrl, c->stream));
+            HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+        } else {
+            HIP_TRY(c, mcml_launch_reset(c->ctrl, kCtrlFirstPass, c->stream));
+            HIP_TRY(c, mcml_launch_first_pass(a, grid, block, c->stream));
+            HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(unsigned long long) * kCtrlFirstPass,
+                                      hipMemcpyDeviceToHost, c->stream));
+        }
         launches++;
-        unsigned long long nr = 0;
-        HIP_TRY(c, hipMemcpyAsync(&nr, c->nrec.p, sizeof(nr), hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         ms_kernel += ms_between(c->ev0, c->ev1);
-        n_rec = nr;
+        n_rec = c->h_ctrl[1];
+        for (int i = 0; i < CNT_N; i++) cnt[i] = c->h_ctrl[2 + i];
         if (n_rec <= cap) break;
         cap = n_rec + n_rec / 8 + 1024;
     }
-    recs.resize((size_t)n_rec);
-    if (n_rec)
-        HIP_TRY(c, hipMemcpy(recs.data(), c->records.p, sizeof(mcml_record) * (size_t)n_rec, hipMemcpyDeviceToHost));
-    std::sort(recs.begin(), recs.end(), rec_less);
+    if (n_rec >= (1ull << 32)) {
+        set_err(c, "more than 2^32 detection records");
+        return MCML_ERR_LIMIT;
+    }
     out->photons = cnt[CNT_PHOTONS];
     out->steps = cnt[CNT_STEPS];
     out->steps_boundary = cnt[CNT_BOUNDARY];
@@ -475,9 +493,9 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
     out->first_pass_launches = launches;
     out->block_size = block;
     out->grid_size = grid;
-    out->chunk = chunk;
-    out->refill_min = refill_min;
-    out->boundary_min = bnd_min;
+    out->chunk = a.chunk;
+    out->refill_min = a.refill_min;
+    out->boundary_min = a.bnd_min;
     if (out->photons != total) {
         set_err(c, "first pass launched %llu photons, expected %llu", (unsigned long long)out->photons,
                 (unsigned long long)total);
@@ -486,123 +504,63 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
     return MCML_OK;
 }
 
-int replay(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opts& o, uint64_t seed,
-           const std::vector<mcml_record>& recs, mcml_result* out)
+/* Sort the records by (set, SDS, photon) on the device. */
+int sort_records(mcml_ctx* c, uint64_t n)
 {
-    const int nsets = (int)der.size();
-    const int S = der[0].num_sds, L = der[0].num_layers;
-    const int nr = der[0].grid_nr, nz = der[0].grid_nz;
-    /* jobs grouped by (set, sds), photon order inside a group: each group is one
-     * pathlength_SDS_# file image */
-    std::vector<ReplayJob> jobs(recs.size());
-    for (size_t i = 0; i < recs.size(); i++) {
-        jobs[i].k = recs[i].key & ((1ull << 40) - 1);
-        jobs[i].set = (uint32_t)(recs[i].key >> 40);
-        jobs[i].sds = recs[i].sds;
-    }
-    std::stable_sort(jobs.begin(), jobs.end(), [](const ReplayJob& a, const ReplayJob& b) {
-        return a.set < b.set || (a.set == b.set && (a.sds < b.sds || (a.sds == b.sds && a.k < b.k)));
-    });
-    const uint64_t nj = jobs.size();
-    const size_t groups = (size_t)nsets * (size_t)S;
-    out->pl_offset = (uint64_t*)calloc(groups + 1, sizeof(uint64_t));
-    out->replay_reflectance = (double*)calloc(groups, sizeof(double));
-    out->pathlength = (float*)malloc(sizeof(float) * (size_t)std::max<uint64_t>(nj, 1) * (size_t)(L + 2));
-    if (!out->pl_offset || !out->replay_reflectance || !out->pathlength) {
-        set_err(c, "host allocation failed");
-        return MCML_ERR_NOMEM;
-    }
-    for (const ReplayJob& j : jobs) out->pl_offset[(size_t)j.set * S + (j.sds - 1) + 1]++;
-    for (size_t g = 0; g < groups; g++) out->pl_offset[g + 1] += out->pl_offset[g];
-
-    const bool grids = o.absorbance != 0;
-    const size_t rz_cells = grids ? groups * (size_t)nz * (size_t)nr : 0;
-    const size_t z_cells = grids ? groups * (size_t)nz : 0;
     int rc;
-    if ((rc = ensure(c, c->jobs, sizeof(ReplayJob) * (size_t)nj)) || (rc = ensure(c, c->rw, sizeof(float) * (size_t)nj)) ||
-        (rc = ensure(c, c->rs, sizeof(float) * (size_t)nj)) ||
-        (rc = ensure(c, c->rpl, sizeof(float) * (size_t)nj * (size_t)L)) ||
-        (rc = ensure(c, c->dump, sizeof(float) * (size_t)nj * (size_t)(L + 2))))
+    if ((rc = ensure(c, c->skey, sizeof(uint64_t) * (size_t)n)) ||
+        (rc = ensure(c, c->sidx, sizeof(uint32_t) * (size_t)n)))
         return rc;
-    if (grids) {
-        if ((rc = ensure(c, c->arz, sizeof(unsigned long long) * rz_cells)) ||
-            (rc = ensure(c, c->a0z, sizeof(unsigned long long) * z_cells)))
-            return rc;
-        HIP_TRY(c, hipMemsetAsync(c->arz.p, 0, sizeof(unsigned long long) * rz_cells, c->stream));
-        HIP_TRY(c, hipMemsetAsync(c->a0z.p, 0, sizeof(unsigned long long) * z_cells, c->stream));
-    }
-    if (nj) {
-        HIP_TRY(c, hipMemcpyAsync(c->jobs.p, jobs.data(), sizeof(ReplayJob) * (size_t)nj, hipMemcpyHostToDevice,
-                                  c->stream));
-        HIP_TRY(c, hipMemsetAsync(c->queue.p, 0, sizeof(unsigned long long), c->stream));
-        ReplayArgs a;
-        memset(&a, 0, sizeof(a));
-        a.sets = as<DevSet>(c->sets);
-        a.jobs = as<ReplayJob>(c->jobs);
-        a.njobs = nj;
-        a.seed = seed;
-        a.mult = c->d_mult;
-        a.queue = as<unsigned long long>(c->queue);
-        a.chunk = 64;
-        a.num_layers = L;
-        a.bnd_min = o.boundary_min > 0 ? std::min(o.boundary_min, 64) : kDefaultBoundary;
-        a.weight = as<float>(c->rw);
-        a.steps = as<float>(c->rs);
-        a.pl = as<float>(c->rpl);
-        a.A_rz = grids ? as<unsigned long long>(c->arz) : nullptr;
-        a.A0_z = grids ? as<unsigned long long>(c->a0z) : nullptr;
-        a.num_sds = S;
-        a.nr = nr;
-        a.nz = nz;
-        a.dr = der[0].grid_dr;
-        a.dz = der[0].grid_dz;
-        int occ = 0;
-        HIP_TRY(c, mcml_occupancy_replay(&occ));
-        if (occ < 1) occ = 1;
-        const uint64_t waves_needed = (nj + 63) / 64;
-        const int per_block = mcml_replay_block() / 64;
-        uint64_t grid = std::min<uint64_t>((uint64_t)c->cu_count * occ, (waves_needed + per_block - 1) / per_block);
-        if (grid < 1) grid = 1;
-        HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-        HIP_TRY(c, mcml_launch_replay(a, (int)grid, c->stream));
-        HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        out->ms_replay_kernel = ms_between(c->ev0, c->ev1);
+    if (n == 0) return MCML_OK;
+    size_t temp = 0;
+    HIP_TRY(c, mcml_sort_keys(nullptr, temp, as<uint64_t>(c->rec_key), as<uint64_t>(c->skey), as<uint32_t>(c->sidx), n,
+                              c->stream));
+    if ((rc = ensure(c, c->stemp, temp))) return rc;
+    HIP_TRY(c, mcml_sort_keys(c->stemp.p, temp, as<uint64_t>(c->rec_key), as<uint64_t>(c->skey), as<uint32_t>(c->sidx),
+                              n, c->stream));
+    return MCML_OK;
+}
 
-        DumpArgs d;
-        memset(&d, 0, sizeof(d));
-        d.weight = a.weight;
-        d.steps = a.steps;
-        d.pl = a.pl;
-        d.njobs = nj;
-        d.num_layers = L;
-        d.out = as<float>(c->dump);
-        const uint64_t elems = nj * (uint64_t)(L + 2);
-        const uint64_t dgrid = std::min<uint64_t>((elems + 255) / 256, (uint64_t)c->cu_count * 16);
+/* Gather rows/weights into sorted order (the dump), build public records, and reduce per
+ * (set, SDS) group; sums and offsets land in host arrays. */
+int gather_and_reduce(mcml_ctx* c, uint64_t n, int nsets, int S, const float* rows_in, int stride_in,
+                      const uint32_t* order, int stride_out, bool records, double* sums_host,
+                      std::vector<uint64_t>& offsets_host, double* dump_ms)
+{
+    int rc;
+    const size_t groups = (size_t)nsets * (size_t)S;
+    if ((rc = ensure(c, c->weights, sizeof(float) * (size_t)n)) ||
+        (rc = ensure(c, c->offsets, sizeof(uint64_t) * (groups + 1))) ||
+        (rc = ensure(c, c->sums, sizeof(double) * groups)))
+        return rc;
+    if (stride_out > 0 && (rc = ensure(c, c->rows, sizeof(float) * (size_t)n * (size_t)stride_out))) return rc;
+    if (records && (rc = ensure(c, c->precs, sizeof(mcml_record) * (size_t)n))) return rc;
+    if (n > 0) {
+        GatherArgs g;
+        memset(&g, 0, sizeof(g));
+        g.sorted_keys = as<uint64_t>(c->skey);
+        g.order = order;
+        g.rows_in = rows_in;
+        g.stride_in = stride_in;
+        g.stride_out = stride_out;
+        g.n = n;
+        g.rows_out = stride_out > 0 ? as<float>(c->rows) : nullptr;
+        g.weight_out = as<float>(c->weights);
+        g.records_out = records ? as<mcml_record>(c->precs) : nullptr;
+        const uint64_t elems = n * (uint64_t)std::max(stride_out, 1);
+        const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((elems + 255) / 256, (uint64_t)c->cu_count * 32));
         HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-        HIP_TRY(c, mcml_launch_dump(d, (int)std::max<uint64_t>(dgrid, 1), c->stream));
+        HIP_TRY(c, mcml_launch_dump(g, (int)grid, c->stream));
         HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(out->pathlength, c->dump.p, sizeof(float) * (size_t)elems, hipMemcpyDeviceToHost,
-                                  c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        out->ms_dump_kernel = ms_between(c->ev0, c->ev1);
     }
-    for (size_t g = 0; g < groups; g++) {
-        double acc = 0;
-        for (uint64_t i = out->pl_offset[g]; i < out->pl_offset[g + 1]; i++) acc += out->pathlength[i * (L + 2)];
-        out->replay_reflectance[g] = acc;
-    }
-    if (grids) {
-        out->A_rz = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(rz_cells, 1));
-        out->A0_z = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(z_cells, 1));
-        if (!out->A_rz || !out->A0_z) {
-            set_err(c, "host allocation failed");
-            return MCML_ERR_NOMEM;
-        }
-        HIP_TRY(c, hipMemcpy(out->A_rz, c->arz.p, sizeof(uint64_t) * rz_cells, hipMemcpyDeviceToHost));
-        HIP_TRY(c, hipMemcpy(out->A0_z, c->a0z.p, sizeof(uint64_t) * z_cells, hipMemcpyDeviceToHost));
-    }
-    out->replayed = 1;
+    HIP_TRY(c, mcml_launch_groups(as<uint64_t>(c->skey), n, nsets, S, as<uint64_t>(c->offsets), as<float>(c->weights),
+                                  as<double>(c->sums), c->stream));
+    offsets_host.resize(groups + 1);
+    HIP_TRY(c, hipMemcpyAsync(offsets_host.data(), c->offsets.p, sizeof(uint64_t) * (groups + 1), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipMemcpyAsync(sums_host, c->sums.p, sizeof(double) * groups, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (dump_ms && n > 0) *dump_ms = ms_between(c->ev0, c->ev1);
     return MCML_OK;
 }
 
@@ -624,12 +582,13 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
     int rc = validate_batch(c, sims, nsets, der);
     if (rc) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
-    const int S = der[0].num_sds;
+    const int S = der[0].num_sds, L = der[0].num_layers;
+    const int nr = der[0].grid_nr, nz = der[0].grid_nz;
     out->nsets = nsets;
     out->num_sds = S;
-    out->num_layers = der[0].num_layers;
-    out->grid_nr = der[0].grid_nr;
-    out->grid_nz = der[0].grid_nz;
+    out->num_layers = L;
+    out->grid_nr = nr;
+    out->grid_nz = nz;
     const size_t groups = (size_t)nsets * (size_t)S;
     out->reflectance = (double*)calloc(groups, sizeof(double));
     out->detected = (uint64_t*)calloc(groups, sizeof(uint64_t));
@@ -637,33 +596,109 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
         mcml_result_free(out);
         return MCML_ERR_NOMEM;
     }
-    std::vector<mcml_record> recs;
-    double t0 = now_ms();
-    rc = first_pass(c, der, o, seed, out, recs);
-    if (rc) {
+    /* replay without absorbance maps: record the pathlengths in the first pass itself */
+    const bool inline_pl = o.replay && !o.absorbance;
+    const int stride = inline_pl ? L + 2 : 1;
+    const double t0 = now_ms();
+    uint64_t n = 0;
+    std::vector<uint64_t> off;
+    double dump_ms = 0;
+    if ((rc = first_pass(c, der, o, seed, stride, out, n)) || (rc = sort_records(c, n)) ||
+        (rc = gather_and_reduce(c, n, nsets, S, as<float>(c->rec_row), stride, as<uint32_t>(c->sidx),
+                                inline_pl ? L + 2 : 0, !o.skip_records, out->reflectance, off, &dump_ms))) {
         mcml_result_free(out);
         return rc;
     }
-    for (const mcml_record& r : recs) { /* calculate_reflectance, transport.cu:299-311 */
-        const size_t g = (size_t)(r.key >> 40) * S + (r.sds - 1);
-        out->reflectance[g] += (double)r.weight;
-        out->detected[g]++;
+    for (size_t g = 0; g < groups; g++) out->detected[g] = off[g + 1] - off[g];
+    out->n_records = n;
+    if (!o.skip_records) {
+        out->records = (mcml_record*)malloc(sizeof(mcml_record) * (size_t)std::max<uint64_t>(n, 1));
+        if (!out->records) {
+            mcml_result_free(out);
+            return MCML_ERR_NOMEM;
+        }
+        if (n)
+            HIP_TRY(c, hipMemcpy(out->records, c->precs.p, sizeof(mcml_record) * (size_t)n, hipMemcpyDeviceToHost));
     }
-    out->n_records = recs.size();
-    out->records = (mcml_record*)malloc(sizeof(mcml_record) * std::max<size_t>(recs.size(), 1));
-    if (!out->records) {
-        mcml_result_free(out);
-        return MCML_ERR_NOMEM;
-    }
-    if (!recs.empty()) memcpy(out->records, recs.data(), sizeof(mcml_record) * recs.size());
-    double t1 = now_ms();
+    const double t1 = now_ms();
     out->ms_first_pass_wall = t1 - t0;
     if (o.replay) {
-        rc = replay(c, der, o, seed, recs, out);
-        if (rc) {
+        out->pl_offset = (uint64_t*)malloc(sizeof(uint64_t) * (groups + 1));
+        out->replay_reflectance = (double*)calloc(groups, sizeof(double));
+        out->pathlength = (float*)malloc(sizeof(float) * (size_t)std::max<uint64_t>(n, 1) * (size_t)(L + 2));
+        if (!out->pl_offset || !out->replay_reflectance || !out->pathlength) {
             mcml_result_free(out);
-            return rc;
+            return MCML_ERR_NOMEM;
         }
+        memcpy(out->pl_offset, off.data(), sizeof(uint64_t) * (groups + 1));
+        if (inline_pl) { /* the first pass already produced the replay rows, in file order */
+            memcpy(out->replay_reflectance, out->reflectance, sizeof(double) * groups);
+            out->ms_dump_kernel = dump_ms;
+            if (n)
+                HIP_TRY(c, hipMemcpy(out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2),
+                                     hipMemcpyDeviceToHost));
+        } else { /* absorbance maps need the true replay (transport.cu:160-222) */
+            const size_t rz_cells = groups * (size_t)nz * (size_t)nr, z_cells = groups * (size_t)nz;
+            if ((rc = ensure(c, c->rrows, sizeof(float) * (size_t)n * (size_t)(L + 2))) ||
+                (rc = ensure(c, c->arz, sizeof(unsigned long long) * rz_cells)) ||
+                (rc = ensure(c, c->a0z, sizeof(unsigned long long) * z_cells))) {
+                mcml_result_free(out);
+                return rc;
+            }
+            HIP_TRY(c, hipMemsetAsync(c->arz.p, 0, sizeof(unsigned long long) * rz_cells, c->stream));
+            HIP_TRY(c, hipMemsetAsync(c->a0z.p, 0, sizeof(unsigned long long) * z_cells, c->stream));
+            if (n) {
+                HIP_TRY(c, hipMemsetAsync(c->ctrl + kCtrlReplayQueue, 0, sizeof(unsigned long long), c->stream));
+                ReplayArgs a;
+                memset(&a, 0, sizeof(a));
+                a.sets = as<DevSet>(c->sets);
+                a.keys = as<uint64_t>(c->skey);
+                a.njobs = n;
+                a.seed = seed;
+                a.mult = c->d_mult;
+                a.queue = c->ctrl + kCtrlReplayQueue;
+                a.chunk = 64;
+                a.num_layers = L;
+                a.bnd_min = o.boundary_min > 0 ? std::min(o.boundary_min, 64) : kDefaultBoundary;
+                a.rows = as<float>(c->rrows);
+                a.A_rz = as<unsigned long long>(c->arz);
+                a.A0_z = as<unsigned long long>(c->a0z);
+                a.num_sds = S;
+                a.nr = nr;
+                a.nz = nz;
+                a.dr = der[0].grid_dr;
+                a.dz = der[0].grid_dz;
+                int occ = 0;
+                HIP_TRY(c, mcml_occupancy_replay(&occ));
+                if (occ < 1) occ = 1;
+                const uint64_t per_block = (uint64_t)mcml_replay_block();
+                const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cu_count * occ,
+                                                                             (n + per_block - 1) / per_block));
+                HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+                HIP_TRY(c, mcml_launch_replay(a, (int)grid, c->stream));
+                HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+                HIP_TRY(c, hipStreamSynchronize(c->stream));
+                out->ms_replay_kernel = ms_between(c->ev0, c->ev1);
+            }
+            std::vector<uint64_t> off2;
+            if ((rc = gather_and_reduce(c, n, nsets, S, as<float>(c->rrows), L + 2, nullptr, L + 2, false,
+                                        out->replay_reflectance, off2, &out->ms_dump_kernel))) {
+                mcml_result_free(out);
+                return rc;
+            }
+            if (n)
+                HIP_TRY(c, hipMemcpy(out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2),
+                                     hipMemcpyDeviceToHost));
+            out->A_rz = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(rz_cells, 1));
+            out->A0_z = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(z_cells, 1));
+            if (!out->A_rz || !out->A0_z) {
+                mcml_result_free(out);
+                return MCML_ERR_NOMEM;
+            }
+            HIP_TRY(c, hipMemcpy(out->A_rz, c->arz.p, sizeof(uint64_t) * rz_cells, hipMemcpyDeviceToHost));
+            HIP_TRY(c, hipMemcpy(out->A0_z, c->a0z.p, sizeof(uint64_t) * z_cells, hipMemcpyDeviceToHost));
+        }
+        out->replayed = 1;
         out->ms_replay_wall = now_ms() - t1;
     }
     return MCML_OK;
@@ -698,5 +733,54 @@ extern "C" int mcml_average_pathlength(const mcml_sim* sim, const mcml_result* r
             outv[s * L + l - 1] = log(sum_r / sum_c) / changed_mua;
         }
     }
+    return MCML_OK;
+}
+
+/* ====================================================================== autotuning */
+extern "C" int mcml_autotune(mcml_ctx* c, const mcml_sim* sims, int nsets, uint64_t trial_photons, mcml_opts* opts)
+{
+    if (!c || !sims || nsets < 1 || !opts || trial_photons < 64) return MCML_ERR_ARG;
+    uint64_t total = 0;
+    for (int j = 0; j < nsets; j++) total += sims[j].number_photons;
+    if (total == 0) return MCML_ERR_ARG;
+    /* the same sets with budgets scaled to about trial_photons in total */
+    std::vector<mcml_sim> trial(sims, sims + nsets);
+    const double scale = std::min(1.0, (double)trial_photons / (double)total);
+    for (mcml_sim& s : trial) s.number_photons = std::max<uint64_t>(1, (uint64_t)((double)s.number_photons * scale));
+    std::vector<mcml_derived> der;
+    int rc = validate_batch(c, trial.data(), nsets, der);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int blocks[] = {128, 256};
+    const int refills[] = {1, 2, 4};
+    const int chunks[] = {64, 128, 256};
+    double best = 1e300;
+    mcml_opts pick = *opts;
+    bool warmed = false;
+    for (int b : blocks)
+        for (int r : refills)
+            for (int ch : chunks) {
+                mcml_opts o;
+                memset(&o, 0, sizeof(o));
+                o.block_size = b;
+                o.refill_min = r;
+                o.chunk = ch;
+                o.boundary_min = opts->boundary_min;
+                for (int rep = warmed ? 0 : -1; rep < 1; rep++) { /* one untimed warm-up run */
+                    mcml_result res;
+                    memset(&res, 0, sizeof(res));
+                    uint64_t nrec = 0;
+                    rc = first_pass(c, der, o, 12345, 1, &res, nrec);
+                    if (rc) return rc;
+                    if (rep == 0 && res.ms_first_pass_kernel < best) {
+                        best = res.ms_first_pass_kernel;
+                        pick.block_size = b;
+                        pick.refill_min = r;
+                        pick.chunk = ch;
+                    }
+                }
+                warmed = true;
+            }
+    *opts = pick;
     return MCML_OK;
 }

@@ -63,7 +63,11 @@ struct Photon {
     int32_t first;
     mcml_mwc rng;
     /* the current layer's parameters, reloaded only when the layer changes */
-    float zmin, zmax, mutr, albedo, g, hg_a, hg_b, hg_c, hg_d, hg_inv2g, n;
+    float zmin, zmax, mutr, albedo, hg_a, hg_b, hg_c, hg_d, hg_inv2g; /* hg_c = 2g: 0 = isotropic */
+    /* running pathlength in the current layer; the per-layer totals live in LDS and are
+     * exchanged with this register only when the photon changes layer */
+    float pl_cur;
+    int32_t pl_layer; /* the tissue layer pl_cur belongs to */
 };
 
 __device__ __forceinline__ void load_layer(Photon& p, const DevSet* S)
@@ -74,13 +78,11 @@ __device__ __forceinline__ void load_layer(Photon& p, const DevSet* S)
     p.zmax = a.y;
     p.mutr = a.z;
     p.albedo = a.w;
-    p.g = b.x;
     p.hg_a = b.y;
     p.hg_b = b.z;
     p.hg_c = b.w;
     p.hg_d = c.x;
     p.hg_inv2g = c.y;
-    p.n = c.z;
 }
 
 /* Launch photon k of set `set` (LaunchPhoton, transport.cu:624-719). */
@@ -122,6 +124,8 @@ __device__ __forceinline__ void launch(Photon& p, const DevSet* S, uint64_t seed
     p.first = 1;
     p.steps = 0;
     p.w = S->start_weight;
+    p.pl_cur = 0.0f;
+    p.pl_layer = 1;
     load_layer(p, S);
 }
 
@@ -132,7 +136,7 @@ __device__ __forceinline__ void launch(Photon& p, const DevSet* S, uint64_t seed
 __device__ __forceinline__ void spin(Photon& p)
 {
     float cost;
-    if (p.g == 0.0f) {
+    if (p.hg_c == 0.0f) { /* g == 0 */
         cost = 2.0f * mcml_uniform(&p.rng) - 1.0f;
     } else {
         const float temp = p.hg_a / (p.hg_b + p.hg_c * mcml_uniform(&p.rng));
@@ -170,7 +174,7 @@ __device__ __forceinline__ bool reflect(Photon& p, const DevSet* S, int new_laye
     const DevLayer& Lc = S->layer[p.layer];
     const bool up = new_layer < p.layer;
     const float4 B = reinterpret_cast<const float4*>(&Lc)[up ? 3 : 4];
-    const float n1 = p.n;
+    const float n1 = Lc.n;
     const float n2 = B.x;
     const float ci = fabsf(p.dz);
     float r;
@@ -272,7 +276,7 @@ __device__ __forceinline__ bool hop(Photon& p, float& s, int& nl, float* pl_lane
     p.y += p.dy * s;
     p.z += p.dz * s;
     p.steps++;
-    if (REPLAY) pl_lane[(p.layer - 1) * pl_stride] += s; /* transport.cu:529 */
+    if (REPLAY) p.pl_cur += s; /* transport.cu:529: pathlength of the layer the hop started in */
     if (p.z > p.zmax) p.z = p.zmax;
     if (p.z < p.zmin) p.z = p.zmin;
     return nl != p.layer;
@@ -341,7 +345,9 @@ __device__ __forceinline__ int interact(Photon& p, float s, GridLane& gl, const 
  * into the next tissue layer; then the survival test.  `target` = 0 tests every detector
  * (first pass), otherwise that detector only (replay).  On ST_DETECTED *mask holds the
  * accepting detectors (bit i = SDS i). */
-__device__ __forceinline__ int cross(Photon& p, int nl, const DevSet* S, int target, uint32_t& mask)
+template <bool PL>
+__device__ __forceinline__ int cross(Photon& p, int nl, const DevSet* S, int target, uint32_t& mask, float* pl_lane,
+                                     int pl_stride)
 {
     const int temp_layer = p.layer;
     if (!reflect(p, S, nl)) {
@@ -358,6 +364,11 @@ __device__ __forceinline__ int cross(Photon& p, int nl, const DevSet* S, int tar
         } else if (nl > S->num_layers) {
             return ST_DEAD;
         } else {
+            if (PL) { /* the same sequence of float additions per layer as pl[layer-1] += s */
+                pl_lane[(p.pl_layer - 1) * pl_stride] = p.pl_cur;
+                p.pl_cur = pl_lane[(nl - 1) * pl_stride];
+                p.pl_layer = nl;
+            }
             load_layer(p, S);
         }
     }
@@ -384,8 +395,9 @@ struct WaveQueue {
 };
 
 __device__ __forceinline__ uint64_t take_ids(WaveQueue& q, uint64_t need, bool mine, unsigned long long* counter,
-                                             uint64_t chunk, uint64_t total)
+                                             uint64_t chunk, uint64_t total, uint64_t& claimed)
 {
+    claimed = ~0ull;
     const int n = __popcll(need);
     const uint32_t rank = rank_in(need);
     const uint64_t avail = q.end - q.next;
@@ -400,6 +412,8 @@ __device__ __forceinline__ uint64_t take_ids(WaveQueue& q, uint64_t need, bool m
         if (base >= total) {
             q.dry = true;
             cend = base;
+        } else {
+            claimed = base;
         }
         if (mine && rank >= avail) {
             const uint64_t t = base + (rank - avail);
@@ -424,13 +438,23 @@ __device__ __forceinline__ bool resolve_now(bool pending, bool alive, int bnd_mi
     return np >= bnd_min || pend == __ballot(alive);
 }
 
-} // namespace
-
-/* ----------------------------------------------------------------- first pass */
-extern "C" __global__ void __launch_bounds__(256) mcml_first_pass(FirstPassArgs a)
+/* Set owning ticket t, using the wavefront's cached set range [clo, chi) first. */
+__device__ __forceinline__ uint32_t set_for(const uint64_t* set_begin, int nsets, uint64_t t, uint32_t cset,
+                                            uint64_t clo, uint64_t chi)
 {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
+    if (nsets == 1) return 0u;
+    if (t >= clo && t < chi) return cset;
+    return set_of_ticket(set_begin, nsets, t);
+}
+
+/* First pass over all photon tickets (MCd + LaunchPhoton_Global + calculate_reflectance,
+ * transport.cu:341-471, 722-739, 299-311).  PL = also accumulate per-layer pathlength and
+ * step count, so that each detection record is already the replay's output row. */
+template <bool PL>
+__device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned char* lds_raw)
+{
     const DevSet* sets = a.sets;
+    size_t lds_off = 0;
     if (a.lds_sets) { /* stage the (small) set table in LDS; reads go through one pointer */
         const uint4* src = reinterpret_cast<const uint4*>(a.sets);
         uint4* dst = reinterpret_cast<uint4*>(lds_raw);
@@ -438,14 +462,22 @@ extern "C" __global__ void __launch_bounds__(256) mcml_first_pass(FirstPassArgs 
         for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
         __syncthreads();
         sets = reinterpret_cast<const DevSet*>(lds_raw);
+        lds_off = sizeof(DevSet) * (size_t)a.nsets;
     }
+    float* pl_lane = reinterpret_cast<float*>(lds_raw + lds_off) + threadIdx.x;
+    const int pl_stride = blockDim.x;
+    const int L = a.num_layers;
     Photon p;
     bool alive = false, pending = false;
     int pend_layer = 0;
-    const DevSet* S = sets;
-    uint64_t key = 0;
+    uint32_t set = 0;
+    uint64_t key = 0; /* set << 45 | photon index */
     WaveQueue q = {0, 0, false};
-    uint64_t c_photons = 0, c_steps = 0, c_bnd = 0, c_int = 0;
+    uint32_t cset = 0;
+    uint64_t clo = 0, chi = 0;
+    /* per-lane work counters; a lane runs at most a few thousand photons of at most
+     * 10 000 steps, far below 2^32 */
+    uint32_t c_photons = 0, c_steps = 0, c_bnd = 0, c_int = 0;
     uint32_t n_int = 0;
     GridLane gl = {nullptr, nullptr, ~0ull, 0};
     const int refill_min = a.refill_min < 1 ? 1 : a.refill_min;
@@ -455,13 +487,20 @@ extern "C" __global__ void __launch_bounds__(256) mcml_first_pass(FirstPassArgs 
         const uint64_t dead = __ballot(!alive);
         const int ndead = __popcll(dead);
         if (!q.dry && (ndead >= refill_min || ndead == 64)) {
-            const uint64_t t = take_ids(q, dead, !alive, a.queue, (uint64_t)a.chunk, a.total);
+            uint64_t claimed;
+            const uint64_t t = take_ids(q, dead, !alive, a.queue, (uint64_t)a.chunk, a.total, claimed);
+            if (claimed != ~0ull && a.nsets > 1) { /* wave-uniform: cache the new chunk's set */
+                cset = set_of_ticket(a.set_begin, a.nsets, claimed);
+                clo = a.set_begin[cset];
+                chi = a.set_begin[cset + 1];
+            }
             if (t != ~0ull) {
-                const uint32_t set = a.nsets > 1 ? set_of_ticket(a.set_begin, a.nsets, t) : 0u;
-                S = sets + set;
+                set = set_for(a.set_begin, a.nsets, t, cset, clo, chi);
                 const uint64_t k = t - a.set_begin[set];
-                key = ((uint64_t)set << 40) | k;
-                launch(p, S, a.seed, set, k, a.mult);
+                key = ((uint64_t)set << MCML_KEY_SET_SHIFT) | k;
+                launch(p, sets + set, a.seed, set, k, a.mult);
+                if (PL)
+                    for (int l = 0; l < L; l++) pl_lane[l * pl_stride] = 0.0f;
                 alive = true;
                 c_photons++;
             }
@@ -472,7 +511,7 @@ extern "C" __global__ void __launch_bounds__(256) mcml_first_pass(FirstPassArgs 
         if (alive && !pending) {
             float s;
             int nl;
-            if (hop<false>(p, s, nl, nullptr, 0)) {
+            if (hop<PL>(p, s, nl, pl_lane, pl_stride)) {
                 pending = true;
                 pend_layer = nl;
             } else {
@@ -482,25 +521,29 @@ extern "C" __global__ void __launch_bounds__(256) mcml_first_pass(FirstPassArgs 
         if (resolve_now(pending, alive, bnd_min) && pending) {
             pending = false;
             c_bnd++;
-            st = cross(p, pend_layer, S, 0, mask);
+            st = cross<PL>(p, pend_layer, sets + set, 0, mask, pl_lane, pl_stride);
         }
         if (st != ST_ALIVE) {
             alive = false;
-            c_steps += (uint64_t)p.steps;
+            if (PL) pl_lane[(p.pl_layer - 1) * pl_stride] = p.pl_cur;
+            c_steps += (uint32_t)p.steps;
             c_int += n_int;
             n_int = 0;
             if (st == ST_DETECTED) {
                 const unsigned long long base = atomicAdd(a.n_records, (unsigned long long)__popc(mask));
                 unsigned long long idx = base;
+                const DevSet* S = sets + set;
                 const int ns = S->num_sds;
                 for (int i = 1; i <= ns; i++) {
                     if (!(mask & (1u << i))) continue;
                     if (idx < a.cap) {
-                        mcml_record r;
-                        r.key = key;
-                        r.sds = (uint32_t)i;
-                        r.weight = credited(p, S, i);
-                        a.records[idx] = r;
+                        a.rec_key[idx] = key | ((uint64_t)i << MCML_KEY_SDS_SHIFT);
+                        float* row = a.rec_row + idx * (uint64_t)a.row_stride;
+                        row[0] = credited(p, S, i);
+                        if (PL) {
+                            for (int l = 0; l < L; l++) row[1 + l] = pl_lane[l * pl_stride];
+                            row[L + 1] = (float)p.steps;
+                        }
                     }
                     idx++;
                 }
@@ -508,12 +551,34 @@ extern "C" __global__ void __launch_bounds__(256) mcml_first_pass(FirstPassArgs 
         }
     }
     const uint64_t s0 = wave_sum(c_photons), s1 = wave_sum(c_steps), s2 = wave_sum(c_bnd), s3 = wave_sum(c_int);
-    if (lane_id() == 0) {
+    if (lane_id() == 0) { /* wave_sum widens to 64 bits before adding */
         atomicAdd(&a.counters[CNT_PHOTONS], (unsigned long long)s0);
         atomicAdd(&a.counters[CNT_STEPS], (unsigned long long)s1);
         atomicAdd(&a.counters[CNT_BOUNDARY], (unsigned long long)s2);
         atomicAdd(&a.counters[CNT_INTERACTION], (unsigned long long)s3);
     }
+}
+
+} // namespace
+
+/* ----------------------------------------------------------------- first pass */
+/* Eight wavefronts per SIMD: the transport is VALU bound and needs every resident wave to
+ * hide the latency of its dependent chains; the few uniform values that do not fit in 96
+ * SGPRs are kept in VGPR lanes. */
+#define FIRST_PASS_ATTRS __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+
+extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass(FirstPassArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    first_pass_body<false>(a, lds_raw);
+}
+
+/* The same pass recording per-layer pathlength (replay without absorbance maps needs no
+ * second pass: a detected photon's first-pass trajectory is its replay trajectory). */
+extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_pl(FirstPassArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    first_pass_body<true>(a, lds_raw);
 }
 
 /* ----------------------------------------------------------------- replay */
@@ -531,7 +596,7 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
     bool alive = false, pending = false;
     int pend_layer = 0;
     uint64_t job = 0;
-    ReplayJob J = {0, 0, 0};
+    uint32_t jset = 0, jsds = 1;
     const DevSet* S = a.sets;
     GridLane gl = {nullptr, nullptr, ~0ull, 0};
     uint32_t n_int = 0;
@@ -539,17 +604,20 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
     for (;;) {
         const uint64_t idle = __ballot(!alive);
         if (!q.dry && idle != 0) {
-            const uint64_t t = take_ids(q, idle, !alive, a.queue, (uint64_t)a.chunk, a.njobs);
+            uint64_t claimed;
+            const uint64_t t = take_ids(q, idle, !alive, a.queue, (uint64_t)a.chunk, a.njobs, claimed);
             if (t != ~0ull) {
                 job = t;
-                J = a.jobs[job];
-                S = a.sets + J.set;
-                launch(p, S, a.seed, J.set, J.k, a.mult);
+                const uint64_t key = a.keys[job];
+                jset = (uint32_t)(key >> MCML_KEY_SET_SHIFT);
+                jsds = (uint32_t)((key >> MCML_KEY_SDS_SHIFT) & 31u);
+                S = a.sets + jset;
+                launch(p, S, a.seed, jset, key & MCML_KEY_K_MASK, a.mult);
                 for (int l = 0; l < L; l++) pl_lane[l * REPLAY_BLOCK] = 0.0f;
                 gl.rl_idx = ~0ull;
                 gl.rl_w = 0;
                 if (a.A_rz != nullptr) {
-                    const uint64_t slab = (uint64_t)J.set * (uint64_t)a.num_sds + (uint64_t)(J.sds - 1);
+                    const uint64_t slab = (uint64_t)jset * (uint64_t)a.num_sds + (uint64_t)(jsds - 1);
                     gl.a_rz = a.A_rz + slab * (uint64_t)a.nz * (uint64_t)a.nr;
                     gl.a0_z = a.A0_z + slab * (uint64_t)a.nz;
                 }
@@ -571,43 +639,184 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
         }
         if (resolve_now(pending, alive, bnd_min) && pending) {
             pending = false;
-            st = cross(p, pend_layer, S, (int)J.sds, mask);
+            st = cross<true>(p, pend_layer, S, (int)jsds, mask, pl_lane, REPLAY_BLOCK);
         }
         if (st != ST_ALIVE) {
             alive = false;
+            pl_lane[(p.pl_layer - 1) * REPLAY_BLOCK] = p.pl_cur;
             if (gl.a_rz != nullptr && gl.rl_w != 0) atomicAdd(&gl.a_rz[gl.rl_idx], (unsigned long long)gl.rl_w);
-            a.weight[job] = (st == ST_DETECTED) ? credited(p, S, (int)J.sds) : 0.0f;
-            a.steps[job] = (float)p.steps;
-            for (int l = 0; l < L; l++) a.pl[(uint64_t)l * a.njobs + job] = pl_lane[l * REPLAY_BLOCK];
+            float* row = a.rows + job * (uint64_t)(L + 2);
+            row[0] = (st == ST_DETECTED) ? credited(p, S, (int)jsds) : 0.0f;
+            for (int l = 0; l < L; l++) row[1 + l] = pl_lane[l * REPLAY_BLOCK];
+            row[L + 1] = (float)p.steps;
         }
     }
 }
 
-/* ----------------------------------------------------------------- dump */
-/* Thread t writes float t of the record image: record t / (L+2), field t % (L+2). */
-extern "C" __global__ void __launch_bounds__(256) mcml_dump(DumpArgs a)
+/* ----------------------------------------------------------------- sorted records */
+/* Pathlength dump into file order: thread t writes float t of the sorted record image
+ * (record t / stride, field t % stride), reading the source row named by `order`; the
+ * stores are fully coalesced.  Also writes the sorted weights and public records. */
+extern "C" __global__ void __launch_bounds__(256) mcml_dump(GatherArgs a)
 {
-    const uint64_t stride = (uint64_t)a.num_layers + 2u;
-    const uint64_t total = a.njobs * stride;
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t rec = t / stride;
-        const uint32_t f = (uint32_t)(t - rec * stride);
-        float v;
-        if (f == 0) v = a.weight[rec];
-        else if (f <= (uint32_t)a.num_layers) v = a.pl[(uint64_t)(f - 1) * a.njobs + rec];
-        else v = a.steps[rec];
-        a.out[t] = v;
+    const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.stride_out > 0) {
+        const uint64_t so = (uint64_t)a.stride_out;
+        const uint64_t total = a.n * so;
+        for (uint64_t t = t0; t < total; t += gsz) {
+            const uint64_t r = t / so;
+            const uint64_t f = t - r * so;
+            const uint64_t src = a.order != nullptr ? (uint64_t)a.order[r] : r;
+            a.rows_out[t] = a.rows_in[src * (uint64_t)a.stride_in + f];
+        }
     }
+    for (uint64_t i = t0; i < a.n; i += gsz) {
+        const uint64_t src = a.order != nullptr ? (uint64_t)a.order[i] : i;
+        const float w = a.rows_in[src * (uint64_t)a.stride_in];
+        a.weight_out[i] = w;
+        if (a.records_out != nullptr) {
+            const uint64_t key = a.sorted_keys[i];
+            mcml_record r;
+            r.key = ((key >> MCML_KEY_SET_SHIFT) << 40) | (key & MCML_KEY_K_MASK);
+            r.sds = (uint32_t)((key >> MCML_KEY_SDS_SHIFT) & 31u);
+            r.weight = w;
+            a.records_out[i] = r;
+        }
+    }
+}
+
+/* Start offset of every (set, SDS) group in the sorted keys (lower bound per group). */
+extern "C" __global__ void __launch_bounds__(256) mcml_group_offsets(const uint64_t* keys, uint64_t n, int nsets,
+                                                                      int nsds, uint64_t* offsets)
+{
+    const int groups = nsets * nsds;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g <= groups; g += gridDim.x * blockDim.x) {
+        uint64_t target;
+        if (g == groups) target = ~0ull;
+        else {
+            const uint64_t set = (uint64_t)(g / nsds), sds = (uint64_t)(g % nsds) + 1u;
+            target = (set << MCML_KEY_SET_SHIFT) | (sds << MCML_KEY_SDS_SHIFT);
+        }
+        uint64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < target) lo = mid + 1;
+            else hi = mid;
+        }
+        offsets[g] = (g == groups) ? n : lo;
+    }
+}
+
+/* Sum of detected weight per (set, SDS) group, in FP64 with a fixed reduction order (one
+ * workgroup per group), so the result does not depend on scheduling. */
+#define GROUP_BLOCK 256
+extern "C" __global__ void __launch_bounds__(GROUP_BLOCK) mcml_group_sums(const float* w, const uint64_t* offsets,
+                                                                          double* sums)
+{
+    __shared__ double part[GROUP_BLOCK];
+    const int g = blockIdx.x;
+    const uint64_t b = offsets[g], e = offsets[g + 1];
+    double acc = 0.0;
+    for (uint64_t i = b + threadIdx.x; i < e; i += GROUP_BLOCK) acc += (double)w[i];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = GROUP_BLOCK / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[g] = part[0];
+}
+
+/* ----------------------------------------------------------------- counters reset */
+/* Zero the control block [queue, n_records, counters...] with a kernel rather than a
+ * memset, so the same sequence can be recorded in a HIP graph. */
+extern "C" __global__ void mcml_reset_ctrl(unsigned long long* ctrl, int n)
+{
+    if ((int)threadIdx.x < n) ctrl[threadIdx.x] = 0ull;
 }
 
 /* ----------------------------------------------------------------- host launchers */
 #include "mcml_launch.h"
 
+hipError_t mcml_launch_reset(unsigned long long* ctrl, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(mcml_reset_ctrl, dim3(1), dim3(64), 0, st, ctrl, n);
+    return hipGetLastError();
+}
+
+/* First pass as a graph: reset kernel -> transport kernel -> read-back of the control
+ * block into pinned host memory.  Rebuilt when the launch shape or a buffer changes,
+ * otherwise only the two kernel nodes' arguments are replaced before each launch. */
+static size_t first_pass_lds(const FirstPassArgs& a, int block)
+{
+    size_t lds = a.lds_sets ? sizeof(DevSet) * (size_t)a.nsets : 0;
+    if (a.row_stride > 1) lds += sizeof(float) * (size_t)a.num_layers * (size_t)block;
+    return lds;
+}
+
+hipError_t mcml_graph_first_pass(FirstPassGraph& G, const FirstPassArgs& a, int grid, int block,
+                                 unsigned long long* ctrl, int nctrl, void* host_ctrl, hipStream_t st)
+{
+    const size_t lds = first_pass_lds(a, block);
+    hipError_t e;
+    FirstPassArgs args = a;
+    unsigned long long* ctrl_arg = ctrl;
+    int n_arg = nctrl;
+    void* fp_params[] = {(void*)&args};
+    void* rs_params[] = {(void*)&ctrl_arg, (void*)&n_arg};
+    hipKernelNodeParams kp = {};
+    kp.func = a.row_stride > 1 ? (void*)mcml_first_pass_pl : (void*)mcml_first_pass;
+    kp.gridDim = dim3(grid);
+    kp.blockDim = dim3(block);
+    kp.sharedMemBytes = (unsigned int)lds;
+    kp.kernelParams = fp_params;
+    kp.extra = nullptr;
+    hipKernelNodeParams rp = {};
+    rp.func = (void*)mcml_reset_ctrl;
+    rp.gridDim = dim3(1);
+    rp.blockDim = dim3(64);
+    rp.sharedMemBytes = 0;
+    rp.kernelParams = rs_params;
+    rp.extra = nullptr;
+    const bool same = G.exec != nullptr && G.grid == grid && G.block == block && G.lds == lds && G.ctrl == ctrl &&
+                      G.host_ctrl == host_ctrl && G.nctrl == nctrl && G.func == kp.func;
+    if (!same) {
+        mcml_graph_release(G);
+        if ((e = hipGraphCreate(&G.graph, 0)) != hipSuccess) return e;
+        if ((e = hipGraphAddKernelNode(&G.reset, G.graph, nullptr, 0, &rp)) != hipSuccess) return e;
+        if ((e = hipGraphAddKernelNode(&G.kernel, G.graph, &G.reset, 1, &kp)) != hipSuccess) return e;
+        if ((e = hipGraphAddMemcpyNode1D(&G.readback, G.graph, &G.kernel, 1, host_ctrl, ctrl,
+                                         sizeof(unsigned long long) * (size_t)nctrl, hipMemcpyDeviceToHost)) !=
+            hipSuccess)
+            return e;
+        if ((e = hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0)) != hipSuccess) return e;
+        G.grid = grid;
+        G.block = block;
+        G.lds = lds;
+        G.ctrl = ctrl;
+        G.host_ctrl = host_ctrl;
+        G.nctrl = nctrl;
+        G.func = kp.func;
+    } else {
+        if ((e = hipGraphExecKernelNodeSetParams(G.exec, G.reset, &rp)) != hipSuccess) return e;
+        if ((e = hipGraphExecKernelNodeSetParams(G.exec, G.kernel, &kp)) != hipSuccess) return e;
+    }
+    return hipGraphLaunch(G.exec, st);
+}
+
+void mcml_graph_release(FirstPassGraph& G)
+{
+    if (G.exec) (void)hipGraphExecDestroy(G.exec);
+    if (G.graph) (void)hipGraphDestroy(G.graph);
+    G = FirstPassGraph();
+}
+
 hipError_t mcml_launch_first_pass(const FirstPassArgs& a, int grid, int block, hipStream_t st)
 {
-    const size_t lds = a.lds_sets ? sizeof(DevSet) * (size_t)a.nsets : 0;
-    hipLaunchKernelGGL(mcml_first_pass, dim3(grid), dim3(block), lds, st, a);
+    const size_t lds = first_pass_lds(a, block);
+    if (a.row_stride > 1) hipLaunchKernelGGL(mcml_first_pass_pl, dim3(grid), dim3(block), lds, st, a);
+    else hipLaunchKernelGGL(mcml_first_pass, dim3(grid), dim3(block), lds, st, a);
     return hipGetLastError();
 }
 
@@ -617,15 +826,30 @@ hipError_t mcml_launch_replay(const ReplayArgs& a, int grid, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t mcml_launch_dump(const DumpArgs& a, int grid, hipStream_t st)
+hipError_t mcml_launch_dump(const GatherArgs& a, int grid, hipStream_t st)
 {
     hipLaunchKernelGGL(mcml_dump, dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
-hipError_t mcml_occupancy_first_pass(int block, size_t lds_bytes, int* blocks_per_cu)
+hipError_t mcml_launch_groups(const uint64_t* keys, uint64_t n, int nsets, int nsds, uint64_t* offsets,
+                              const float* w, double* sums, hipStream_t st)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_first_pass, block, lds_bytes);
+    const int groups = nsets * nsds;
+    hipLaunchKernelGGL(mcml_group_offsets, dim3((groups + 1 + 255) / 256), dim3(256), 0, st, keys, n, nsets, nsds,
+                       offsets);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mcml_group_sums, dim3(groups), dim3(GROUP_BLOCK), 0, st, w, (const uint64_t*)offsets, sums);
+    return hipGetLastError();
+}
+
+hipError_t mcml_occupancy_first_pass(const FirstPassArgs& a, int block, int* blocks_per_cu)
+{
+    const size_t lds = first_pass_lds(a, block);
+    if (a.row_stride > 1)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_first_pass_pl, block, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_first_pass, block, lds);
 }
 
 hipError_t mcml_occupancy_replay(int* blocks_per_cu)
