@@ -224,6 +224,44 @@ int mcml_autotune(mcml_ctx* ctx, const mcml_sim* sims, int nsets, uint64_t trial
  * dmua = 0.001 mua (transport.cu:274-297).  out: [S*L] (SDS-major). */
 int mcml_average_pathlength(const mcml_sim* sim, const mcml_result* res, int set, double* out);
 
+/* White Monte Carlo rescaling of a replayed result (res->replayed): for every set j and
+ * each of its m new absorption vectors mua_new[(j*m + i)*L + l] (1/cm, layer l+1), the
+ * detected weight per SDS the same photons would carry, w * exp(-sum_l (mua'_l - mua_l) L_l),
+ * summed per (set, SDS) into out[(j*m + i)*S + sds-1] (W-units, like res->reflectance).
+ * sims are the sets the result was simulated with.  The analogue of the reference's
+ * perturbation estimate in calculate_average_pathlength (transport.cu:285-292), here for
+ * arbitrary mua.  out_sq (optional, same shape) receives the sums of squared rescaled
+ * weights, from which a label's standard error follows. */
+int mcml_white_rescale(mcml_ctx* ctx, const mcml_sim* sims, const mcml_result* res, const float* mua_new, int m,
+                       double* out, double* out_sq);
+
+/* ---------------------------------------------------------------------------------
+ * Reflectance surrogate network (no counterpart in the reference).  A fully connected
+ * network with widths[0] inputs, GELU hidden layers widths[1..n-2] and widths[n-1] linear
+ * outputs; matrix products on the matrix cores in bf16 with FP32 accumulation, FP32 master
+ * weights, Adam.  Parameters: per layer W (out x in, row-major) then b (out).
+ * --------------------------------------------------------------------------------- */
+typedef struct mcml_mlp mcml_mlp;
+mcml_mlp* mcml_mlp_create(int device, const int* widths, int nwidths, uint64_t seed, int* err);
+void mcml_mlp_destroy(mcml_mlp* mlp);
+int64_t mcml_mlp_num_params(const mcml_mlp* mlp);
+/* write = 0: copy parameters out; write = 1: load them (and reset the Adam state). */
+int mcml_mlp_params(mcml_mlp* mlp, float* host, int64_t n, int write);
+/* y[B x out] = network(x[B x in]); *ms = kernel time of the forward pass (HIP events). */
+int mcml_mlp_predict(mcml_mlp* mlp, const float* x, int B, float* y, double* ms);
+/* Gradient of mean_b mean_o w_bo (y_bo - t_bo)^2 at the current parameters; w holds one
+ * weight per target [B x out] and may be NULL (all ones).  *loss = that mean. */
+int mcml_mlp_gradient(mcml_mlp* mlp, const float* x, const float* t, const float* w, int B, float* grad, float* loss);
+/* One Adam step (beta 0.9/0.999, eps 1e-8; L2 weight decay wd added to the gradient) with
+ * gradient grad. */
+int mcml_mlp_adam_step(mcml_mlp* mlp, const float* grad, float lr, float wd);
+/* Mini-batch Adam over n samples (per-target weights w [n x out] or NULL) for `epochs`
+ * epochs of n / batch steps, cosine learning-rate decay from lr to lr_final, reshuffled
+ * every epoch from seed; epoch_loss[e] = loss of epoch e's last batch (may be NULL);
+ * *ms = device time of the whole run. */
+int mcml_mlp_train(mcml_mlp* mlp, const float* x, const float* t, const float* w, int n, int epochs, int batch,
+                   float lr, float lr_final, float wd, uint64_t seed, float* epoch_loss, double* ms);
+
 /* Read sim_set.json + input.txt (read_mua_mus, io.cpp:35-256).  On success *sims holds
  * *nsims simulations (free with mcml_free_sims).  On rejection the reference's message is
  * copied to msg. */

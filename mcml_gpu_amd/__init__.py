@@ -58,6 +58,8 @@ def load_library():
                                      C.c_char_p, P(abi.SummaryInfo)]
     L.mcml_main.argtypes = [C.c_int, P(C.c_char_p)]
     L.mcml_autotune.argtypes = [C.c_void_p, P(abi.Sim), C.c_int, C.c_uint64, P(abi.Opts)]
+    L.mcml_white_rescale.argtypes = [C.c_void_p, P(abi.Sim), P(abi.Result), P(C.c_float), C.c_int,
+                                     P(C.c_double), P(C.c_double)]
     _lib = L
     return L
 
@@ -192,6 +194,30 @@ class Context:
         o = opts if opts is not None else make_opts()
         self._check(self.lib.mcml_autotune(self.h, arr, len(sims), int(trial_photons), C.byref(o)))
         return o
+
+    def white_rescale(self, sims, result, mua_new, with_squares=False):
+        """White Monte Carlo: detected weight per (set, mua' vector, SDS) that the replayed
+        photons of `result` would carry at absorption `mua_new` [nsets, m, L] (1/cm);
+        same units as RunResult.reflectance.  with_squares also returns the sums of the
+        squared rescaled weights."""
+        if isinstance(sims, abi.Sim):
+            sims = [sims]
+        mua_new = np.ascontiguousarray(mua_new, dtype=np.float32)
+        nsets, m, L = mua_new.shape
+        assert nsets == result.nsets and L == result.num_layers and result.replayed
+        r = abi.Result()
+        r.nsets, r.num_sds, r.num_layers, r.replayed = result.nsets, result.num_sds, result.num_layers, 1
+        off = np.ascontiguousarray(result.pl_offset, dtype=np.uint64)
+        rows = np.ascontiguousarray(result.pathlength, dtype=np.float32)
+        r.pl_offset = off.ctypes.data_as(C.POINTER(C.c_uint64))
+        r.pathlength = rows.ctypes.data_as(C.POINTER(C.c_float))
+        out = np.zeros((nsets, m, result.num_sds), np.float64)
+        sq = np.zeros_like(out) if with_squares else None
+        arr = (abi.Sim * len(sims))(*sims)
+        dp = C.POINTER(C.c_double)
+        self._check(self.lib.mcml_white_rescale(self.h, arr, C.byref(r), mua_new.ctypes.data_as(C.POINTER(C.c_float)),
+                                                m, out.ctypes.data_as(dp), sq.ctypes.data_as(dp) if with_squares else None))
+        return (out, sq) if with_squares else out
 
     def close(self):
         if self.h:

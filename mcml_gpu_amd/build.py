@@ -23,7 +23,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
           "-Wall", "-Wno-unused-result", "-I" + os.path.join(ROOT, "include")]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["transport.hip", "sort.hip", "runtime.cpp", "io.cpp"]
+SOURCES = ["transport.hip", "sort.hip", "white.hip", "mlp.hip", "runtime.cpp", "mlp_host.cpp", "io.cpp"]
 
 
 def _run(cmd):

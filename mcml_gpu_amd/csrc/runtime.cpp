@@ -26,6 +26,7 @@
 #include "mcml_device.h"
 #include "mcml_launch.h"
 #include "mwc_multipliers.h"
+#include "white.h"
 
 namespace {
 
@@ -65,6 +66,7 @@ struct mcml_ctx {
     DevBuf rows, weights, precs;      /* sorted rows (file image), weights, public records */
     DevBuf offsets, sums;             /* per (set, SDS) group: start offsets, FP64 sums */
     DevBuf rrows, arz, a0z;           /* replay rows and absorbance grids */
+    DevBuf wrows, woff, wold, wnew, wout; /* White Monte Carlo inputs and output */
 };
 
 namespace {
@@ -335,7 +337,7 @@ extern "C" void mcml_destroy(mcml_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->sets,  &c->set_begin, &c->rec_key, &c->rec_row, &c->skey,  &c->sidx, &c->stemp,
                       &c->rows,  &c->weights,   &c->precs,   &c->offsets, &c->sums,  &c->rrows, &c->arz,
-                      &c->a0z};
+                      &c->a0z,   &c->wrows,     &c->woff,    &c->wold,    &c->wnew,  &c->wout};
     for (DevBuf* b : bufs) release(*b);
     mcml_graph_release(c->graph);
     if (c->d_mult) (void)hipFree(c->d_mult);
@@ -782,5 +784,58 @@ extern "C" int mcml_autotune(mcml_ctx* c, const mcml_sim* sims, int nsets, uint6
                 warmed = true;
             }
     *opts = pick;
+    return MCML_OK;
+}
+
+/* ====================================================================== White Monte Carlo */
+extern "C" int mcml_white_rescale(mcml_ctx* c, const mcml_sim* sims, const mcml_result* res, const float* mua_new,
+                                  int m, double* out, double* out_sq)
+{
+    if (!c || !sims || !res || !mua_new || !out || m < 1 || !res->replayed) return MCML_ERR_ARG;
+    const int nsets = res->nsets, S = res->num_sds, L = res->num_layers;
+    if (L > 16) return MCML_ERR_LIMIT;
+    if (m > 65535 || (int64_t)nsets * S > 65535) {
+        set_err(c, "at most 65535 mua vectors and 65535 (set, SDS) groups per call");
+        return MCML_ERR_LIMIT;
+    }
+    const size_t groups = (size_t)nsets * (size_t)S;
+    const uint64_t n = res->pl_offset[groups];
+    std::vector<float> old((size_t)nsets * (size_t)L);
+    for (int j = 0; j < nsets; j++)
+        for (int l = 0; l < L; l++) old[(size_t)j * L + l] = sims[j].layers[l].mua;
+    int rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if ((rc = ensure(c, c->wrows, sizeof(float) * (size_t)std::max<uint64_t>(n, 1) * (size_t)(L + 2))) ||
+        (rc = ensure(c, c->woff, sizeof(uint64_t) * (groups + 1))) ||
+        (rc = ensure(c, c->wold, sizeof(float) * old.size())) ||
+        (rc = ensure(c, c->wnew, sizeof(float) * (size_t)nsets * (size_t)m * (size_t)L)) ||
+        (rc = ensure(c, c->wout, 2 * sizeof(double) * (size_t)nsets * (size_t)m * (size_t)S)))
+        return rc;
+    const size_t nout = (size_t)nsets * (size_t)m * (size_t)S;
+    if (n)
+        HIP_TRY(c, hipMemcpyAsync(c->wrows.p, res->pathlength, sizeof(float) * (size_t)n * (size_t)(L + 2),
+                                  hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->woff.p, res->pl_offset, sizeof(uint64_t) * (groups + 1), hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->wold.p, old.data(), sizeof(float) * old.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->wnew.p, mua_new, sizeof(float) * (size_t)nsets * (size_t)m * (size_t)L,
+                              hipMemcpyHostToDevice, c->stream));
+    WhiteArgs a;
+    memset(&a, 0, sizeof(a));
+    a.rows = as<float>(c->wrows);
+    a.offsets = as<uint64_t>(c->woff);
+    a.mua_old = as<float>(c->wold);
+    a.mua_new = as<float>(c->wnew);
+    a.out = as<double>(c->wout);
+    a.out_sq = out_sq ? as<double>(c->wout) + nout : nullptr;
+    a.m = m;
+    a.num_layers = L;
+    a.num_sds = S;
+    HIP_TRY(c, mcml_launch_white(a, nsets, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(out, c->wout.p, sizeof(double) * nout, hipMemcpyDeviceToHost, c->stream));
+    if (out_sq)
+        HIP_TRY(c, hipMemcpyAsync(out_sq, as<double>(c->wout) + nout, sizeof(double) * nout, hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     return MCML_OK;
 }
