@@ -12,7 +12,11 @@ roofline = the first-pass transport kernel, FP32 VALU bound: algorithmic FLOP pe
 kernel's average duration from HIP events on the library's stream.
 cpu_baseline = the CPU restatement of the same loop (oracle, OpenMP) on a bounded sample.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--photons N] [--no-cpu]
+--config cfg3: the 512-set batch in one persistent launch (photons/s).
+--config cfg4: the reflectance surrogate (R evaluations/s of the trained network at its
+measured error against high-photon plain MC).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--photons N] [--config cfgN] [--no-cpu]
 """
 import argparse
 import json
@@ -45,6 +49,13 @@ def main():
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--bpc", type=int, default=0)
     ap.add_argument("--bnd", type=int, default=0)
+    ap.add_argument("--train-sets", type=int, default=16384, help="cfg4: simulated training sets")
+    ap.add_argument("--train-photons", type=float, default=6e4, help="cfg4: photons per training set")
+    ap.add_argument("--white", type=int, default=13, help="cfg4: absorption vectors per set (White MC)")
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--heldout-sets", type=int, default=1000)
+    ap.add_argument("--heldout-photons", type=float, default=1e8)
+    ap.add_argument("--infer-batch", type=int, default=100000)
     ap.add_argument("--calibrate-copy", type=float, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter scale)")
     args = ap.parse_args()
@@ -70,6 +81,8 @@ def main():
 
     if args.config == "cfg3":
         return bench_batch(args)
+    if args.config == "cfg4":
+        return bench_surrogate(args)
     sim = M.load_config(args.config)
     sim.number_photons = int(args.photons)
     opts = M.make_opts(replay=True, pathlength=True, binary=True, block_size=args.block,
@@ -198,6 +211,101 @@ def bench_batch(args):
                      "traffic": None},
     }
     print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+MLP_FLOP_PER_SAMPLE = 2 * (14 * 256 + 3 * 256 * 256 + 256 * 3)  # forward multiply-adds x 2
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def bench_surrogate(args):
+    """cfg4: train the 14 -> 4x256 GELU -> 3 network on White-MC-expanded batched MC, judge it
+    on held-out sets simulated with plain MC, and time batched inference."""
+    import numpy as np
+    import mcml_gpu_amd as M
+    from mcml_gpu_amd import batch
+    from mcml_gpu_amd import surrogate as SG
+
+    ctx = M.Context(0)
+    t0 = time.perf_counter()
+    params = batch.draw_parameters(args.train_sets, seed=1)
+    xs, Rs, rvs = [], [], []
+    per = 1024
+    for i in range(0, args.train_sets, per):
+        p = params[i:i + per]
+        x, R, rv = SG.white_dataset(ctx, p, np.full(len(p), int(args.train_photons)), args.white, seed=1 + i)
+        xs.append(x), Rs.append(R), rvs.append(rv)
+    x, R, rv = np.concatenate(xs), np.concatenate(Rs), np.concatenate(rvs)
+    t_data = time.perf_counter() - t0
+    print("training samples %d from %d sets x %.3g photons (%.1f s)" % (len(x), args.train_sets, args.train_photons,
+                                                                       t_data), flush=True)
+    net, norm, hist = SG.fit(x, R, rv, epochs=args.epochs, batch_size=1024, lr=2e-3, seed=0)
+    print("trained %d epochs in %.1f s (last-batch loss %.3g)" % (args.epochs, net.train_ms / 1e3, hist[-1]),
+          flush=True)
+
+    t1 = time.perf_counter()
+    hp, hR, hrv = SG.reference_set(
+        ctx, args.heldout_sets, args.heldout_photons, seed=9001,
+        progress=lambda i, n: print("held-out MC %d/%d sets (%.0f s)" % (i, n, time.perf_counter() - t1), flush=True))
+    t_ref = time.perf_counter() - t1
+    # plain-MC throughput on these sets (first pass only, no replay), for the MC cost per evaluation
+    n_pl = min(20, args.heldout_sets)
+    _, res_pl, _ = SG.simulate(ctx, hp[:n_pl], np.full(n_pl, int(1e7)), seed=77, replay=False)
+    tp = time.perf_counter()
+    SG.simulate(ctx, hp[:n_pl], np.full(n_pl, int(1e7)), seed=78, replay=False)
+    mc_pps = n_pl * 1e7 / (time.perf_counter() - tp)
+    sur = SG.Surrogate(net, norm, bound=0.02)
+    med = sur.validate(hp, hR)
+    err = SG.relative_error(SG.predict_R(net, norm, hp), hR)
+
+    q = SG.features(batch.draw_parameters(args.infer_batch, seed=4242))
+    for _ in range(max(1, args.warmup)):
+        net.predict(q)
+    kms = []
+    for _ in range(args.steps):
+        net.predict(q)
+        kms.append(net.last_ms)
+    k_ms = float(np.mean(kms))
+    evals_per_s = args.infer_batch / (k_ms * 1e-3)
+    achieved = MLP_FLOP_PER_SAMPLE * args.infer_batch / (k_ms * 1e-3) / 1e12
+    # plain MC cost of one evaluation at 2% relative standard error at every SDS
+    photons_2pct = float(np.median(SG.photons_for_error(hrv * args.heldout_photons, 0.02)))
+    mc_evals_per_s = mc_pps / photons_2pct
+    line = {
+        "metric": "surrogate R-evals/sec at <=2% median rel err vs 1e8-photon MC", "value": evals_per_s,
+        "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": k_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (simulated by this library)",
+        "config": {"workload": "cfg4: MLP 14-4x256 GELU-3, inference batch %d; trained on %d samples (%d sets "
+                               "x %.3g photons, White MC x%d); judged on %d held-out sets x %.3g photons"
+                               % (args.infer_batch, len(x), args.train_sets, args.train_photons, args.white,
+                                  args.heldout_sets, args.heldout_photons),
+                   "epochs": args.epochs},
+        "median_rel_err": med, "median_rel_err_per_sds": [float(v) for v in np.median(err, axis=0)],
+        "p90_rel_err": float(np.quantile(err, 0.9)), "within_bound": bool(sur.usable),
+        "mc_photons_per_s": mc_pps, "mc_photons_for_2pct": photons_2pct, "mc_evals_per_s": mc_evals_per_s,
+        "gain_vs_mc": evals_per_s / mc_evals_per_s, "train_s": net.train_ms / 1e3, "data_s": t_data,
+        "heldout_s": t_ref,
+        "roofline": {"bound": "mfma", "kernel": "mcml_mlp_fused", "achieved": achieved,
+                     "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS,
+                     "flop_per_sample": MLP_FLOP_PER_SAMPLE, "traffic": None},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import mlp_oracle as MO
+        p = net.get_params()
+        n_cpu = 20000
+        tc = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - tc < 10.0:
+            MO.forward(p, SG.WIDTHS, q[:n_cpu], bf16=False)
+            reps += 1
+        dt = time.perf_counter() - tc
+        line["cpu_baseline"] = {"value": reps * n_cpu / dt, "unit": "evals/s",
+                                "cores": int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1),
+                                "kind": "port", "sample": "%d x %d rows, numpy float64 forward" % (reps, n_cpu)}
+    print(json.dumps(line), flush=True)
+    net.close()
     ctx.close()
 
 

@@ -245,9 +245,14 @@ typedef struct mcml_mlp mcml_mlp;
 mcml_mlp* mcml_mlp_create(int device, const int* widths, int nwidths, uint64_t seed, int* err);
 void mcml_mlp_destroy(mcml_mlp* mlp);
 int64_t mcml_mlp_num_params(const mcml_mlp* mlp);
+/* enable = 1/0 allows/forbids the fused inference kernel (-1: leave as is); returns 1 when
+ * predict will use it, 0 when it runs layer by layer, < 0 on error. */
+int mcml_mlp_use_fused(mcml_mlp* mlp, int enable);
 /* write = 0: copy parameters out; write = 1: load them (and reset the Adam state). */
 int mcml_mlp_params(mcml_mlp* mlp, float* host, int64_t n, int write);
-/* y[B x out] = network(x[B x in]); *ms = kernel time of the forward pass (HIP events). */
+/* y[B x out] = network(x[B x in]); *ms = kernel time of the forward pass (HIP events).
+ * Networks of <= 32 inputs, 1..8 GELU layers of 256 and <= 16 outputs run as one fused
+ * kernel (weights staged in LDS, activations in registers); others layer by layer. */
 int mcml_mlp_predict(mcml_mlp* mlp, const float* x, int B, float* y, double* ms);
 /* Gradient of mean_b mean_o w_bo (y_bo - t_bo)^2 at the current parameters; w holds one
  * weight per target [B x out] and may be NULL (all ones).  *loss = that mean. */

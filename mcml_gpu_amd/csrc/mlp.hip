@@ -33,13 +33,26 @@ __device__ __forceinline__ short to_bf16(float x)
     return (short)(u >> 16);
 }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+/* Standard normal CDF Phi(z) = (1 + erf(z / sqrt 2)) / 2 with erf from Abramowitz & Stegun
+ * 7.1.26 (|error| <= 1.5e-7, far below the bf16 rounding the activations then get). */
+__device__ __forceinline__ float normal_cdf(float z)
+{
+    const float x = fabsf(z) * 0.70710678118654752f;
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, x, 1.0f));
+    float p = fmaf(1.061405429f, t, -1.453152027f);
+    p = fmaf(p, t, 1.421413741f);
+    p = fmaf(p, t, -0.284496736f);
+    p = fmaf(p, t, 0.254829592f);
+    p *= t;
+    const float erf_abs = 1.0f - p * __expf(-x * x);
+    return 0.5f + 0.5f * copysignf(erf_abs, z);
+}
+
+__device__ __forceinline__ float gelu(float z) { return z * normal_cdf(z); }
 
 __device__ __forceinline__ float gelu_grad(float z)
 {
-    const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
-    return cdf + z * pdf;
+    return normal_cdf(z) + z * (0.39894228040143268f * __expf(-0.5f * z * z));
 }
 
 } // namespace
@@ -114,14 +127,26 @@ extern "C" __global__ void __launch_bounds__(256) mcml_mlp_reduce(const float* p
     }
 }
 
-/* Column sums of dZ [M x N] (bias gradient): one thread per column, rows in order. */
-extern "C" __global__ void __launch_bounds__(256) mcml_mlp_colsum(const float* dz, int M, int N, float* out)
+/* Partial column sums of dZ [M x N] (bias gradient) over row range blockIdx.y of `rows`
+ * rows: 64 columns per workgroup, four interleaved row slices per column summed with four
+ * independent accumulators, combined in a fixed order.  part[blockIdx.y * N + n]. */
+extern "C" __global__ void __launch_bounds__(256) mcml_mlp_colsum(const float* dz, int M, int N, int rows, float* part)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    float s = 0.0f;
-    for (int m = 0; m < M; m++) s += dz[(int64_t)m * N + n];
-    out[n] = s;
+    __shared__ float slice_sum[4][64];
+    const int c = threadIdx.x & 63, slice = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + c;
+    const int beg = blockIdx.y * rows, end = min(M, beg + rows);
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (n < N) {
+        int m = beg + slice;
+        for (; m + 12 < end; m += 16)
+            for (int u = 0; u < 4; u++) acc[u] += dz[(int64_t)(m + 4 * u) * N + n];
+        for (int u = 0; m < end; m += 4, u++) acc[u & 3] += dz[(int64_t)m * N + n];
+    }
+    slice_sum[slice][c] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    __syncthreads();
+    if (slice == 0 && n < N)
+        part[(int64_t)blockIdx.y * N + n] = (slice_sum[0][c] + slice_sum[1][c]) + (slice_sum[2][c] + slice_sum[3][c]);
 }
 
 /* Weighted squared error with one weight per target: dY = 2 w (Y - T) / (B * outs), and the
@@ -168,6 +193,144 @@ extern "C" __global__ void __launch_bounds__(256) mcml_mlp_gather(const float* s
     }
 }
 
+/* ------------------------------------------------------------------ fused inference
+ * Transposed formulation: each layer computes Z^T = W X^T, so a 16x16 output tile held by
+ * a lane as rows n = 4q + e (q = lane / 16, e = 0..3) of column m = lane % 16 is already
+ * the B operand the next layer's v_mfma_f32_16x16x32_bf16 wants for its k = n: two adjacent
+ * output tiles make one 32-deep k step, with k order {4q + e, 16 + 4q + e} that the packed
+ * weights follow.  Activations therefore stay in registers from the first layer to the
+ * last; each layer's bf16 weights are staged once per workgroup into LDS (128 KB for a
+ * 256 x 256 layer) and every A-operand read there feeds two row blocks.  A workgroup of
+ * eight wavefronts covers 256 rows, 32 per wavefront. */
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+namespace {
+
+constexpr int FW_KT = MLP_FUSED_H / 32; /* 32-deep k steps of a hidden layer */
+constexpr int FW_MB = 2;                /* 16-row blocks per wavefront */
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+__device__ __forceinline__ void stage(uint4* lds, const uint4* g, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
+}
+
+/* One GELU layer from KT_IN k steps of input to MLP_FUSED_H outputs. */
+template <int KT_IN>
+__device__ __forceinline__ void fused_hidden(const uint4* w, const float* __restrict__ bias,
+                                             const bf16x8 (&in)[KT_IN][FW_MB], bf16x8 (&out)[FW_KT][FW_MB], int r, int q)
+{
+#pragma unroll
+    for (int c = 0; c < FW_KT; c++) {
+        f32x4 acc[2][FW_MB];
+#pragma unroll
+        for (int mb = 0; mb < FW_MB; mb++) acc[0][mb] = acc[1][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < KT_IN; t++) {
+            const bf16x8 a0 = as_bf16x8(w[(t * 4 + q) * MLP_FUSED_H + 32 * c + r]);
+            const bf16x8 a1 = as_bf16x8(w[(t * 4 + q) * MLP_FUSED_H + 32 * c + 16 + r]);
+#pragma unroll
+            for (int mb = 0; mb < FW_MB; mb++) {
+                acc[0][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, in[t][mb], acc[0][mb], 0, 0, 0);
+                acc[1][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, in[t][mb], acc[1][mb], 0, 0, 0);
+            }
+        }
+        const float4 b0 = *reinterpret_cast<const float4*>(bias + 32 * c + 4 * q);
+        const float4 b1 = *reinterpret_cast<const float4*>(bias + 32 * c + 16 + 4 * q);
+        const float bb0[4] = {b0.x, b0.y, b0.z, b0.w}, bb1[4] = {b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+        for (int mb = 0; mb < FW_MB; mb++) {
+            bf16x8 v;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                v[e] = (__bf16)gelu(acc[0][mb][e] + bb0[e]);
+                v[4 + e] = (__bf16)gelu(acc[1][mb][e] + bb1[e]);
+            }
+            out[c][mb] = v;
+        }
+    }
+}
+
+} // namespace
+
+extern "C" __global__ void __launch_bounds__(MLP_FUSED_WAVES * 64) mcml_mlp_fused(MlpFusedArgs a)
+{
+    __shared__ uint4 wlds[FW_KT * 4 * MLP_FUSED_H];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const int row0 = blockIdx.x * MLP_FUSED_ROWS + wave * (FW_MB * 16);
+    const uint4* pk = reinterpret_cast<const uint4*>(a.packed);
+
+    /* first layer's B operand straight from x: k = 4q + j (j < 4), 12 + 4q + j (j >= 4) */
+    bf16x8 x0[1][FW_MB];
+#pragma unroll
+    for (int mb = 0; mb < FW_MB; mb++) {
+        const int m = row0 + 16 * mb + r;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int k = 4 * q + (j < 4 ? j : 12 + j);
+            x0[0][mb][j] = (__bf16)((m < a.B && k < a.in) ? a.x[(int64_t)m * a.in + k] : 0.0f);
+        }
+    }
+    stage(wlds, pk + a.w_off[0] / 8, 4 * MLP_FUSED_H);
+    __syncthreads();
+    bf16x8 act[FW_KT][FW_MB];
+    fused_hidden<1>(wlds, a.bias + a.b_off[0], x0, act, r, q);
+    for (int l = 1; l < a.hidden; l++) {
+        __syncthreads();
+        stage(wlds, pk + a.w_off[l] / 8, FW_KT * 4 * MLP_FUSED_H);
+        __syncthreads();
+        bf16x8 nxt[FW_KT][FW_MB];
+        fused_hidden<FW_KT>(wlds, a.bias + a.b_off[l], act, nxt, r, q);
+#pragma unroll
+        for (int t = 0; t < FW_KT; t++)
+#pragma unroll
+            for (int mb = 0; mb < FW_MB; mb++) act[t][mb] = nxt[t][mb];
+    }
+    /* linear output layer, one 16-row block of outputs */
+    __syncthreads();
+    stage(wlds, pk + a.w_off[a.hidden] / 8, FW_KT * 4 * 16);
+    __syncthreads();
+    f32x4 acc[FW_MB];
+#pragma unroll
+    for (int mb = 0; mb < FW_MB; mb++) acc[mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < FW_KT; t++) {
+        const bf16x8 w = as_bf16x8(wlds[(t * 4 + q) * 16 + r]);
+#pragma unroll
+        for (int mb = 0; mb < FW_MB; mb++) acc[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, act[t][mb], acc[mb], 0, 0, 0);
+    }
+    const float* bl = a.bias + a.b_off[a.hidden];
+#pragma unroll
+    for (int mb = 0; mb < FW_MB; mb++) {
+        const int m = row0 + 16 * mb + r;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int n = 4 * q + e;
+            if (m < a.B && n < a.outs) a.y[(int64_t)m * a.outs + n] = acc[mb][e] + bl[n];
+        }
+    }
+}
+
+/* Pack one layer (W [out x in] fp32, b [out]) into the fused kernel's operand order. */
+extern "C" __global__ void __launch_bounds__(256) mcml_mlp_pack(const float* W, const float* b, int in, int out,
+                                                                int Kp, int Np, uint16_t* dst, float* dst_bias)
+{
+    const int64_t total = (int64_t)Kp * Np;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(e & 7);
+        const int n = (int)((e >> 3) % Np);
+        const int64_t tq = (e >> 3) / Np;
+        const int qq = (int)(tq & 3);
+        const int t = (int)(tq >> 2);
+        const int k = 32 * t + 4 * qq + (j < 4 ? j : 12 + j);
+        const float v = (n < out && k < in) ? W[(int64_t)n * in + k] : 0.0f;
+        dst[e] = (uint16_t)to_bf16(v);
+        if (e < Np) dst_bias[e] = e < out ? b[e] : 0.0f;
+    }
+}
+
 /* ------------------------------------------------------------------ launchers */
 hipError_t mcml_mlp_launch_gather(const float* src, const int32_t* idx, int rows, int cols, float* dst, hipStream_t st)
 {
@@ -192,9 +355,10 @@ hipError_t mcml_mlp_launch_reduce(const float* part, int splits, int64_t stride,
     return hipGetLastError();
 }
 
-hipError_t mcml_mlp_launch_colsum(const float* dz, int M, int N, float* out, hipStream_t st)
+hipError_t mcml_mlp_launch_colsum(const float* dz, int M, int N, int splits, float* part, hipStream_t st)
 {
-    hipLaunchKernelGGL(mcml_mlp_colsum, dim3((N + 255) / 256), dim3(256), 0, st, dz, M, N, out);
+    const int rows = (M + splits - 1) / splits;
+    hipLaunchKernelGGL(mcml_mlp_colsum, dim3((N + 63) / 64, splits), dim3(256), 0, st, dz, M, N, rows, part);
     return hipGetLastError();
 }
 
@@ -211,5 +375,21 @@ hipError_t mcml_mlp_launch_adam(float* p, const float* g, float* m1, float* m2, 
     const int64_t blocks = (n + 255) / 256;
     hipLaunchKernelGGL(mcml_mlp_adam, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, st, p, g, m1, m2,
                        n, lr_t, b1, b2, eps, wd);
+    return hipGetLastError();
+}
+
+hipError_t mcml_mlp_launch_pack(const float* W, const float* b, int in, int out, int Kp, int Np, uint16_t* dst,
+                                float* dst_bias, hipStream_t st)
+{
+    const int64_t blocks = ((int64_t)Kp * Np + 255) / 256;
+    hipLaunchKernelGGL(mcml_mlp_pack, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, st, W, b, in, out,
+                       Kp, Np, dst, dst_bias);
+    return hipGetLastError();
+}
+
+hipError_t mcml_mlp_launch_fused(const MlpFusedArgs& a, hipStream_t st)
+{
+    const int blocks = (a.B + MLP_FUSED_ROWS - 1) / MLP_FUSED_ROWS;
+    hipLaunchKernelGGL(mcml_mlp_fused, dim3(blocks), dim3(MLP_FUSED_WAVES * 64), 0, st, a);
     return hipGetLastError();
 }

@@ -58,6 +58,12 @@ struct mcml_mlp {
     Buf x, y, dy, loss, dz0, dz1;
     Buf data_x, data_y, data_w;
     Buf bt, bw;              /* mini-batch targets and weights */
+    /* fused inference: packed bf16 weights, refreshed when the parameters change */
+    bool fused_ok = false, fused_on = true;
+    int64_t version = 0, packed_version = -1;
+    uint16_t* packed = nullptr;
+    Buf packed_b;
+    MlpFusedArgs fa;
     int32_t* idx = nullptr;
     size_t idx_n = 0;
     int cap = 0;             /* batch capacity of the activation buffers */
@@ -93,10 +99,44 @@ int ensure_batch(mcml_mlp* m, int B)
     return MCML_OK;
 }
 
+/* Pack the parameters for the fused kernel if they changed since the last pack. */
+int pack_fused(mcml_mlp* m)
+{
+    if (m->packed_version == m->version) return MCML_OK;
+    const size_t nl = m->layers.size();
+    int64_t w = 0, b = 0;
+    for (size_t l = 0; l < nl; l++) {
+        const Layer& L = m->layers[l];
+        const int Kp = l == 0 ? 32 : MLP_FUSED_H, Np = l + 1 < nl ? MLP_FUSED_H : 16;
+        m->fa.w_off[l] = w;
+        m->fa.b_off[l] = b;
+        MLP_TRY(mcml_mlp_launch_pack(m->params.p + L.w_off, m->params.p + L.b_off, L.in, L.out, Kp, Np, m->packed + w,
+                                     m->packed_b.p + b, m->stream));
+        w += (int64_t)Kp * Np;
+        b += Np;
+    }
+    m->packed_version = m->version;
+    return MCML_OK;
+}
+
 /* Forward pass of B rows held in m->x; keep_z stores pre-activations for backward. */
 int forward(mcml_mlp* m, int B, bool keep_z)
 {
     const size_t nl = m->layers.size();
+    if (!keep_z && m->fused_ok && m->fused_on) {
+        int rc = pack_fused(m);
+        if (rc) return rc;
+        m->fa.x = m->x.p;
+        m->fa.y = m->y.p;
+        m->fa.packed = m->packed;
+        m->fa.bias = m->packed_b.p;
+        m->fa.B = B;
+        m->fa.in = m->layers[0].in;
+        m->fa.outs = m->layers.back().out;
+        m->fa.hidden = (int)nl - 1;
+        MLP_TRY(mcml_mlp_launch_fused(m->fa, m->stream));
+        return MCML_OK;
+    }
     const float* in = m->x.p;
     for (size_t l = 0; l < nl; l++) {
         const Layer& L = m->layers[l];
@@ -133,7 +173,9 @@ int weight_grad(mcml_mlp* m, const Layer& L, const float* dz, const float* a_in,
 {
     const int splits = std::max(1, std::min(64, B / 256));
     const int64_t plane = (int64_t)L.out * L.in;
-    if (grow(m->part, (size_t)plane * splits)) return MCML_ERR_NOMEM;
+    /* weight partials [splits][out][in], then bias partials [splits][out] */
+    if (grow(m->part, (size_t)(plane + L.out) * splits)) return MCML_ERR_NOMEM;
+    float* bias_part = m->part.p + plane * splits;
     MlpGemmArgs g;
     memset(&g, 0, sizeof(g));
     g.A = dz; /* A(o, b) = dZ[b][o] */
@@ -151,7 +193,8 @@ int weight_grad(mcml_mlp* m, const Layer& L, const float* dz, const float* a_in,
     g.epilogue = MLP_EPI_SPLIT;
     MLP_TRY(mcml_mlp_launch_gemm(g, splits, m->stream));
     MLP_TRY(mcml_mlp_launch_reduce(m->part.p, splits, plane, plane, m->grad.p + L.w_off, m->stream));
-    MLP_TRY(mcml_mlp_launch_colsum(dz, B, L.out, m->grad.p + L.b_off, m->stream));
+    MLP_TRY(mcml_mlp_launch_colsum(dz, B, L.out, splits, bias_part, m->stream));
+    MLP_TRY(mcml_mlp_launch_reduce(bias_part, splits, L.out, L.out, m->grad.p + L.b_off, m->stream));
     return MCML_OK;
 }
 
@@ -199,6 +242,7 @@ int adam(mcml_mlp* m, float lr, float wd)
     m->step++;
     const double c1 = 1.0 - std::pow((double)b1, (double)m->step), c2 = 1.0 - std::pow((double)b2, (double)m->step);
     const float lr_t = (float)(lr * std::sqrt(c2) / c1);
+    m->version++;
     MLP_TRY(mcml_mlp_launch_adam(m->params.p, m->grad.p, m->m1.p, m->m2.p, m->nparams, lr_t, b1, b2, eps, wd,
                                  m->stream));
     return MCML_OK;
@@ -232,6 +276,10 @@ extern "C" mcml_mlp* mcml_mlp_create(int device, const int* widths, int nwidths,
         m->layers.push_back(L);
     }
     m->nparams = off;
+    /* the fused inference kernel takes <= 32 inputs, 1..8 GELU layers of 256, <= 16 outputs */
+    m->fused_ok = nwidths >= 3 && nwidths - 2 <= MLP_FUSED_MAX_HIDDEN && widths[0] <= 32 && widths[nwidths - 1] <= 16;
+    for (int l = 1; l + 1 < nwidths; l++) m->fused_ok = m->fused_ok && widths[l] == MLP_FUSED_H;
+    memset(&m->fa, 0, sizeof(m->fa));
     /* He-style initialisation (normal, std sqrt(2/in)), zero biases, from the seed */
     std::vector<float> init((size_t)off, 0.0f);
     std::mt19937_64 rng(seed);
@@ -244,7 +292,11 @@ extern "C" mcml_mlp* mcml_mlp_create(int device, const int* widths, int nwidths,
         grow(m->grad, off) || grow(m->m1, off) || grow(m->m2, off) ||
         hipMemcpy(m->params.p, init.data(), sizeof(float) * off, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(m->m1.p, 0, sizeof(float) * off) != hipSuccess ||
-        hipMemset(m->m2.p, 0, sizeof(float) * off) != hipSuccess) {
+        hipMemset(m->m2.p, 0, sizeof(float) * off) != hipSuccess ||
+        (m->fused_ok && (hipMalloc((void**)&m->packed, sizeof(uint16_t) * (size_t)(32 * MLP_FUSED_H + (nwidths - 3) *
+                                                                                   MLP_FUSED_H * MLP_FUSED_H +
+                                                                               MLP_FUSED_H * 16)) != hipSuccess ||
+                         grow(m->packed_b, (size_t)(nwidths - 2) * MLP_FUSED_H + 16)))) {
         *err = MCML_ERR_DEVICE;
         mcml_mlp_destroy(m);
         return nullptr;
@@ -267,6 +319,8 @@ extern "C" void mcml_mlp_destroy(mcml_mlp* m)
     for (Buf& b : m->a)
         if (b.p) (void)hipFree(b.p);
     if (m->idx) (void)hipFree(m->idx);
+    if (m->packed) (void)hipFree(m->packed);
+    if (m->packed_b.p) (void)hipFree(m->packed_b.p);
     if (m->e0) (void)hipEventDestroy(m->e0);
     if (m->e1) (void)hipEventDestroy(m->e1);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -274,6 +328,13 @@ extern "C" void mcml_mlp_destroy(mcml_mlp* m)
 }
 
 extern "C" int64_t mcml_mlp_num_params(const mcml_mlp* m) { return m ? m->nparams : 0; }
+
+extern "C" int mcml_mlp_use_fused(mcml_mlp* m, int enable)
+{
+    if (!m) return MCML_ERR_ARG;
+    if (enable >= 0) m->fused_on = enable != 0;
+    return m->fused_ok && m->fused_on ? 1 : 0;
+}
 
 extern "C" int mcml_mlp_params(mcml_mlp* m, float* host, int64_t n, int write)
 {
@@ -283,6 +344,7 @@ extern "C" int mcml_mlp_params(mcml_mlp* m, float* host, int64_t n, int write)
     if (write) MLP_TRY(hipMemcpy(m->params.p, host, sizeof(float) * n, k));
     else MLP_TRY(hipMemcpy(host, m->params.p, sizeof(float) * n, k));
     if (write) {
+        m->version++;
         MLP_TRY(hipMemset(m->m1.p, 0, sizeof(float) * n));
         MLP_TRY(hipMemset(m->m2.p, 0, sizeof(float) * n));
         m->step = 0;

@@ -26,6 +26,7 @@ def _bind(L):
     L.mcml_mlp_destroy.argtypes = [C.c_void_p]
     L.mcml_mlp_num_params.argtypes = [C.c_void_p]
     L.mcml_mlp_num_params.restype = C.c_int64
+    L.mcml_mlp_use_fused.argtypes = [C.c_void_p, C.c_int]
     L.mcml_mlp_params.argtypes = [C.c_void_p, P(C.c_float), C.c_int64, C.c_int]
     L.mcml_mlp_predict.argtypes = [C.c_void_p, P(C.c_float), C.c_int, P(C.c_float), P(C.c_double)]
     L.mcml_mlp_gradient.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_float), C.c_int, P(C.c_float),
@@ -57,6 +58,14 @@ class MLP:
     def _check(self, rc, what):
         if rc:
             raise MCMLError(rc, what)
+
+    def use_fused(self, enable=None):
+        """Allow (True) or forbid (False) the fused inference kernel; returns whether predict
+        uses it."""
+        rc = self.lib.mcml_mlp_use_fused(self.h, -1 if enable is None else int(bool(enable)))
+        if rc < 0:
+            raise MCMLError(rc, "use_fused")
+        return bool(rc)
 
     def get_params(self):
         p = np.zeros(self.n, np.float32)

@@ -152,10 +152,14 @@ def mlp_cls():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("widths,B", [(FULL, 1000), (ODD, 70), (FULL, 1), (FULL, 4097)])
-def test_forward_matches_bf16_restatement(mlp_cls, widths, B):
-    print("forward", widths, B, flush=True)
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "layered"])
+@pytest.mark.parametrize("widths,B", [(FULL, 1000), (ODD, 70), (FULL, 1), (FULL, 4097), ((20, 256, 256, 9), 300)])
+def test_forward_matches_bf16_restatement(mlp_cls, widths, B, fused):
+    """Both inference paths: the fused kernel (widths <= 32 -> 256 ... -> <= 16) and the
+    layer-by-layer products."""
+    print("forward", widths, B, "fused" if fused else "layered", flush=True)
     net = mlp_cls(widths, seed=11)
+    assert net.use_fused(fused) == (fused and widths != ODD)
     p = random_params(widths, 12)
     net.set_params(p)
     assert np.array_equal(net.get_params(), p)
@@ -169,6 +173,12 @@ def test_forward_matches_bf16_restatement(mlp_cls, widths, B):
     assert e16 <= 2e-3
     if widths == FULL and B > 1:
         assert e16 <= 0.2 * e64
+    # parameter updates reach the fused kernel's packed copy
+    p2 = random_params(widths, 14)
+    net.set_params(p2)
+    y2 = net.predict(x)
+    y2_16, _, _ = MO.forward(p2, widths, x, bf16=True)
+    assert _rms(y2 - y2_16) / _rms(y2_16) <= 2e-3
 
 
 @pytest.mark.gpu
@@ -234,3 +244,6 @@ def test_training_learns_and_is_deterministic(mlp_cls):
     assert hist[-1] < 0.02 and hist[-1] < hist[0] / 10
     y = net.predict(x[:1000])
     assert _rms(y - t[:1000]) < 0.2
+    net.use_fused(False)
+    y_layered = net.predict(x[:1000])
+    assert _rms(y - y_layered) < 1e-2 * _rms(y)
