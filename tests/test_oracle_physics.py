@@ -328,3 +328,69 @@ def test_slab_reflectance_matches_independent_numpy_mc(n_slab):
     assert abs(R - Rn) < 4 * np.hypot(se, sen), (R, Rn)
     if n_slab == 1.0:
         assert abs(R - 0.09739) < 4 * se, (R, se)
+
+
+# --------------------------------------------------------------------- spatially resolved
+def farrell_disk(pos, radius, mua, mus_p):
+    """Diffuse reflectance of a semi-infinite, index-matched medium for a pencil beam
+    (Farrell, Patterson & Wilson 1992, dipole with extrapolated boundary zb = 2D), integrated
+    over a disk of the given radius centred at distance pos."""
+    mutp = mua + mus_p
+    a, z0, D = mus_p / mutp, 1.0 / mutp, 1.0 / (3.0 * mutp)
+    meff, zb = math.sqrt(3.0 * mua * mutp), 2.0 * D
+
+    def R(rho):
+        r1 = np.sqrt(z0 ** 2 + rho ** 2)
+        r2 = np.sqrt((z0 + 2 * zb) ** 2 + rho ** 2)
+        return a / (4 * np.pi) * (z0 * (meff + 1 / r1) * np.exp(-meff * r1) / r1 ** 2 +
+                                  (z0 + 2 * zb) * (meff + 1 / r2) * np.exp(-meff * r2) / r2 ** 2)
+
+    xs = np.linspace(pos - radius, pos + radius, 601)
+    X, Y = np.meshgrid(xs, xs - pos)
+    inside = (X - pos) ** 2 + Y ** 2 <= radius * radius
+    return float((R(np.hypot(X, Y)) * inside).sum() * (xs[1] - xs[0]) ** 2)
+
+
+def test_spatially_resolved_reflectance_within_diffusion_band():
+    """Ring-mode detection at 5-12 transport mean free paths from a pencil beam against the
+    diffusion dipole: within 15%, approaching it as the distance grows (diffusion theory
+    overestimates close to the source).  The ring weight is the arc fraction of the photon's
+    circle inside each detector disk, so the expectation is the disk integral of R."""
+    n, mua, mus = 400000, 0.1, 10.0
+    dets = [(0.5, 1.0, 0.15, 0.0), (0.8, 1.0, 0.15, 0.0), (1.2, 1.0, 0.15, 0.0)]
+    sim = abi.make_sim(F.base_cfg(n, 1, 1.0, 1.0, 0, 0, F.probes((0, 0, 0), dets)), [(0.0, mua, mus, 1.0, 0.0)])
+    d = O.derive(sim)
+    rec, _ = O.first_pass(d, 5, 0, n)
+    ratios = []
+    for s, (pos, _, r, _) in enumerate(dets, 1):
+        m = rec["sds"] == s
+        est = rec["weight"][m].astype(np.float64).sum() / (4294967295.0 * n)
+        ratios.append(est / farrell_disk(pos, r, mua, mus))
+    assert all(0.8 < q < 1.1 for q in ratios), ratios
+    assert ratios[0] < ratios[1] < ratios[2] and abs(ratios[2] - 1) < 0.06, ratios
+
+
+# --------------------------------------------------------------------- absorbance grids
+def test_replay_deposits_balance_weight_loss():
+    """A replayed photon that never met roulette deposits in A_rz and A0_z exactly the u64
+    drops it lost, so their sum equals start weight - final weight up to the float rounding
+    of w -= drop (under 256 W-units per step at these weights)."""
+    n = 3000
+    sim = F.hemisphere(n, mua=1.0, mus=100.0, g=0.9, n=1.4)
+    sim.grid_nr, sim.grid_nz, sim.grid_dr, sim.grid_dz = 20, 20, 0.05, 0.05
+    d = O.derive(sim)
+    rec, _ = O.first_pass(d, 99, 0, n)
+    start = float(d.start_weight)
+    checked = 0
+    for key in rec["key"][:300]:
+        arz = np.zeros(400, np.uint64)
+        a0 = np.zeros(20, np.uint64)
+        hit, r = O.replay(d, 99, int(key), 1, grids=(arz, a0))
+        assert hit == 1
+        final, steps = float(r[0]), float(r[-1])
+        if final <= 10 * 429497:  # roulette may have multiplied the weight
+            continue
+        deposited = int(arz.sum()) + int(a0.sum())
+        assert abs(deposited - (start - final)) <= 256 * steps + 1, (deposited, start - final, steps)
+        checked += 1
+    assert checked > 100
