@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--bpc", type=int, default=0)
     ap.add_argument("--bnd", type=int, default=0)
+    ap.add_argument("--pathlength", action="store_true",
+                    help="cfg3: also record per-layer pathlengths and dump the records (-R -P)")
     ap.add_argument("--train-sets", type=int, default=16384, help="cfg4: simulated training sets")
     ap.add_argument("--train-photons", type=float, default=6e4, help="cfg4: photons per training set")
     ap.add_argument("--white", type=int, default=13, help="cfg4: absorption vectors per set (White MC)")
@@ -182,15 +184,17 @@ def bench_batch(args):
     ctx = M.Context(0)
     tuned = ctx.autotune(sims, trial_photons=2e7)
     opts = M.make_opts(block_size=tuned.block_size, chunk=tuned.chunk, refill_min=tuned.refill_min,
-                       use_graph=True, skip_records=True)
+                       use_graph=True, skip_records=not args.pathlength, replay=args.pathlength,
+                       pathlength=args.pathlength, binary=args.pathlength)
     for _ in range(args.warmup):
         ctx.run_batch(sims, opts, seed=args.seed)
-    kern, walls_fp = [], []
+    kern, walls_fp, dumps = [], [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         res = ctx.run_batch(sims, opts, seed=args.seed + i)
         kern.append(res.ms_first_pass_kernel)
         walls_fp.append(res.ms_first_pass_wall)
+        dumps.append(res.ms_dump_kernel)
     elapsed = time.perf_counter() - t0
     k_ms = float(np.mean(kern))
     flop = (FLOP_INTERACTION * res.steps_interaction + FLOP_BOUNDARY * res.steps_boundary)
@@ -210,6 +214,16 @@ def bench_batch(args):
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": None},
     }
+    if args.pathlength:
+        n_rec = int(res.detected.sum())
+        stride = sims[0].number_layers + 2
+        # per record: sorted key 8 + source index 4 + source row read + row write + weight 4 + public record 16
+        dump_bytes = n_rec * (8 + 4 + 2 * 4 * stride + 4 + 16)
+        d_ms = float(np.mean(dumps))
+        line["config"]["workload"] += ", -R -P: per-layer pathlength records dumped in file order"
+        line["dump"] = {"kernel": "mcml_dump", "records": n_rec, "bytes": dump_bytes, "kernel_ms": d_ms,
+                        "GB_per_s": dump_bytes / (d_ms * 1e-3) / 1e9 if d_ms > 0 else None,
+                        "frac_of_hbm_peak": dump_bytes / (d_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if d_ms > 0 else None}
     print(json.dumps(line), flush=True)
     ctx.close()
 

@@ -48,7 +48,21 @@ __device__ __forceinline__ float normal_cdf(float z)
     return 0.5f + 0.5f * copysignf(erf_abs, z);
 }
 
-__device__ __forceinline__ float gelu(float z) { return z * normal_cdf(z); }
+/* GELU(z) = z Phi(z) with the same erf approximation, rearranged for fewer operations:
+ * z (1 + sign(z) erf(|z|/sqrt 2)) / 2 = max(z, 0) - |z| (poly(t) / 2) e^{-z^2/2}, where
+ * t = 1 / (1 + 0.3275911 |z| / sqrt 2); the 1/2 and 1/sqrt 2 are folded into constants. */
+__device__ __forceinline__ float gelu(float z)
+{
+    const float az = fabsf(z);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f * 0.70710678118654752f, az, 1.0f));
+    float p = fmaf(0.5f * 1.061405429f, t, 0.5f * -1.453152027f);
+    p = fmaf(p, t, 0.5f * 1.421413741f);
+    p = fmaf(p, t, 0.5f * -0.284496736f);
+    p = fmaf(p, t, 0.5f * 0.254829592f);
+    p *= t;
+    const float e = __expf(z * (-0.5f * z));
+    return fmaf(-az, p * e, fmaxf(z, 0.0f));
+}
 
 __device__ __forceinline__ float gelu_grad(float z)
 {
@@ -211,9 +225,18 @@ constexpr int FW_MB = 2;                /* 16-row blocks per wavefront */
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-__device__ __forceinline__ void stage(uint4* lds, const uint4* g, int n)
+/* Copy N 16-byte words of packed weights into LDS: every load is issued before the first
+ * store, so the copy costs one memory latency instead of one per word. */
+template <int N>
+__device__ __forceinline__ void stage(uint4* lds, const uint4* g)
 {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
+    constexpr int T = MLP_FUSED_WAVES * 64, PER = N / T;
+    static_assert(N % T == 0, "stage size must be a multiple of the workgroup");
+    uint4 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) v[i] = g[threadIdx.x + i * T];
+#pragma unroll
+    for (int i = 0; i < PER; i++) lds[threadIdx.x + i * T] = v[i];
 }
 
 /* One GELU layer from KT_IN k steps of input to MLP_FUSED_H outputs. */
@@ -262,24 +285,36 @@ extern "C" __global__ void __launch_bounds__(MLP_FUSED_WAVES * 64) mcml_mlp_fuse
     const int row0 = blockIdx.x * MLP_FUSED_ROWS + wave * (FW_MB * 16);
     const uint4* pk = reinterpret_cast<const uint4*>(a.packed);
 
-    /* first layer's B operand straight from x: k = 4q + j (j < 4), 12 + 4q + j (j >= 4) */
+    /* first layer's B operand straight from x: k = 4q + j (j < 4), 12 + 4q + j (j >= 4).
+     * Addresses are clamped into the array and out-of-range values zeroed afterwards, so
+     * all sixteen loads are in flight together (B >= 1 and in >= 1). */
+    float xv[FW_MB][8];
+#pragma unroll
+    for (int mb = 0; mb < FW_MB; mb++) {
+        const int m = min(row0 + 16 * mb + r, a.B - 1);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int k = min(4 * q + (j < 4 ? j : 12 + j), a.in - 1);
+            xv[mb][j] = a.x[(int64_t)m * a.in + k];
+        }
+    }
     bf16x8 x0[1][FW_MB];
 #pragma unroll
     for (int mb = 0; mb < FW_MB; mb++) {
-        const int m = row0 + 16 * mb + r;
+        const bool row_in = row0 + 16 * mb + r < a.B;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const int k = 4 * q + (j < 4 ? j : 12 + j);
-            x0[0][mb][j] = (__bf16)((m < a.B && k < a.in) ? a.x[(int64_t)m * a.in + k] : 0.0f);
+            const bool k_in = 4 * q + (j < 4 ? j : 12 + j) < a.in;
+            x0[0][mb][j] = (__bf16)((row_in && k_in) ? xv[mb][j] : 0.0f);
         }
     }
-    stage(wlds, pk + a.w_off[0] / 8, 4 * MLP_FUSED_H);
+    stage<4 * MLP_FUSED_H>(wlds, pk + a.w_off[0] / 8);
     __syncthreads();
     bf16x8 act[FW_KT][FW_MB];
     fused_hidden<1>(wlds, a.bias + a.b_off[0], x0, act, r, q);
     for (int l = 1; l < a.hidden; l++) {
         __syncthreads();
-        stage(wlds, pk + a.w_off[l] / 8, FW_KT * 4 * MLP_FUSED_H);
+        stage<FW_KT * 4 * MLP_FUSED_H>(wlds, pk + a.w_off[l] / 8);
         __syncthreads();
         bf16x8 nxt[FW_KT][FW_MB];
         fused_hidden<FW_KT>(wlds, a.bias + a.b_off[l], act, nxt, r, q);
@@ -290,7 +325,7 @@ extern "C" __global__ void __launch_bounds__(MLP_FUSED_WAVES * 64) mcml_mlp_fuse
     }
     /* linear output layer, one 16-row block of outputs */
     __syncthreads();
-    stage(wlds, pk + a.w_off[a.hidden] / 8, FW_KT * 4 * 16);
+    stage<FW_KT * 4 * 16>(wlds, pk + a.w_off[a.hidden] / 8);
     __syncthreads();
     f32x4 acc[FW_MB];
 #pragma unroll
