@@ -549,8 +549,8 @@ int gather_and_reduce(mcml_ctx* c, uint64_t n, int nsets, int S, const float* ro
         g.rows_out = stride_out > 0 ? as<float>(c->rows) : nullptr;
         g.weight_out = as<float>(c->weights);
         g.records_out = records ? as<mcml_record>(c->precs) : nullptr;
-        const uint64_t elems = n * (uint64_t)std::max(stride_out, 1);
-        const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((elems + 255) / 256, (uint64_t)c->cu_count * 32));
+        /* one 256-record tile per workgroup pass; the grid covers the chip several times over */
+        const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)c->cu_count * 16));
         HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
         HIP_TRY(c, mcml_launch_dump(g, (int)grid, c->stream));
         HIP_TRY(c, hipEventRecord(c->ev1, c->stream));

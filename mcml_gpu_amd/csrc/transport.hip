@@ -298,13 +298,39 @@ __device__ __forceinline__ int survive(Photon& p)
 }
 
 /* Absorbance bookkeeping of one replayed photon (transport.cu:580-600): the first drop goes
- * to A0_z, later ones to A_rz, merged while consecutive drops stay in one cell. */
+ * to A0_z, later ones to A_rz, merged while consecutive drops stay in one cell.
+ *
+ * Each wavefront owns an LDS tile of the hot corner of one (set, SDS) group's grids (z bins
+ * < TILE_Z, and r bins < TILE_R for A_rz), accumulated with u64 LDS atomics and flushed to
+ * the global grids with u64 atomics when the wavefront moves on to another group and at the
+ * end.  Drops outside the corner, or of a lane whose photon belongs to another group, go
+ * straight to the global grids.  u64 sums are exact and associative, so the grids do not
+ * depend on where a drop was first added (the reference's own accumulation type,
+ * transport.cu:335-339; FP32 tiles would round). */
+#define TILE_Z 16
+#define TILE_R 16
+#define TILE_WORDS (TILE_Z * TILE_R + TILE_Z)
+
 struct GridLane {
-    unsigned long long* a_rz;
+    unsigned long long* a_rz;   /* this lane's (set, SDS) slabs */
     unsigned long long* a0_z;
-    uint64_t rl_idx;
+    unsigned long long* tile;   /* the wavefront's LDS tile (NULL without grids) */
+    uint64_t rl_key;            /* run in progress: z bin << 32 | r bin */
     uint64_t rl_w;
+    uint32_t grp;               /* this lane's (set, SDS) group: set * S + sds - 1 */
+    uint32_t tile_grp;          /* the group the tile holds (wave-uniform; ~0 = none) */
 };
+
+__device__ __forceinline__ void flush_run(GridLane& gl, int nr)
+{
+    if (gl.rl_w == 0) return;
+    const uint32_t zb = (uint32_t)(gl.rl_key >> 32), rb = (uint32_t)gl.rl_key;
+    if (gl.grp == gl.tile_grp && zb < TILE_Z && rb < TILE_R)
+        atomicAdd(&gl.tile[zb * TILE_R + rb], (unsigned long long)gl.rl_w);
+    else
+        atomicAdd(&gl.a_rz[(uint64_t)zb * (uint64_t)nr + rb], (unsigned long long)gl.rl_w);
+    gl.rl_w = 0;
+}
 
 /* Interaction (transport.cu:430-437 / 573-603): drop weight and scatter when the hop did
  * not end on a boundary and s > 0, then the survival test. */
@@ -320,16 +346,19 @@ __device__ __forceinline__ int interact(Photon& p, float s, GridLane& gl, const 
             qz = fminf(fmaxf(qz, 0.0f), (float)(ra->nz - 1));
             const int zb = (int)qz;
             if (p.first) {
-                atomicAdd(&gl.a0_z[zb], (unsigned long long)w_temp);
+                if (gl.grp == gl.tile_grp && zb < TILE_Z)
+                    atomicAdd(&gl.tile[TILE_Z * TILE_R + zb], (unsigned long long)w_temp);
+                else
+                    atomicAdd(&gl.a0_z[zb], (unsigned long long)w_temp);
             } else {
                 float qr = sqrtf(p.x * p.x + p.y * p.y) / ra->dr;
                 qr = fminf(fmaxf(qr, 0.0f), (float)(ra->nr - 1));
-                const uint64_t idx = (uint64_t)zb * (uint64_t)ra->nr + (uint64_t)(int)qr;
-                if (idx == gl.rl_idx) {
+                const uint64_t key = ((uint64_t)(uint32_t)zb << 32) | (uint32_t)(int)qr;
+                if (key == gl.rl_key) {
                     gl.rl_w += (uint64_t)w_temp;
                 } else {
-                    if (gl.rl_w != 0) atomicAdd(&gl.a_rz[gl.rl_idx], (unsigned long long)gl.rl_w);
-                    gl.rl_idx = idx;
+                    flush_run(gl, ra->nr);
+                    gl.rl_key = key;
                     gl.rl_w = (uint64_t)w_temp;
                 }
             }
@@ -479,7 +508,7 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
      * 10 000 steps, far below 2^32 */
     uint32_t c_photons = 0, c_steps = 0, c_bnd = 0, c_int = 0;
     uint32_t n_int = 0;
-    GridLane gl = {nullptr, nullptr, ~0ull, 0};
+    GridLane gl = {nullptr, nullptr, nullptr, ~0ull, 0, 0, ~0u};
     const int refill_min = a.refill_min < 1 ? 1 : a.refill_min;
     const int bnd_min = a.bnd_min < 1 ? 1 : a.bnd_min;
 
@@ -584,10 +613,29 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_pl(FirstPassArgs a)
 /* ----------------------------------------------------------------- replay */
 #define REPLAY_BLOCK 256
 
+/* Add a wavefront's tile into its group's global grids and clear it (wave-cooperative). */
+__device__ __forceinline__ void flush_tile(unsigned long long* tile, uint32_t grp, const ReplayArgs& a)
+{
+    if (grp == ~0u) return;
+    unsigned long long* rz = a.A_rz + (uint64_t)grp * (uint64_t)a.nz * (uint64_t)a.nr;
+    unsigned long long* z0 = a.A0_z + (uint64_t)grp * (uint64_t)a.nz;
+    for (int i = lane_id(); i < TILE_WORDS; i += 64) {
+        const unsigned long long v = tile[i];
+        if (v == 0ull) continue;
+        if (i < TILE_Z * TILE_R)
+            atomicAdd(&rz[(uint64_t)(i / TILE_R) * (uint64_t)a.nr + (uint64_t)(i % TILE_R)], v);
+        else
+            atomicAdd(&z0[i - TILE_Z * TILE_R], v);
+        tile[i] = 0ull;
+    }
+}
+
+
 extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArgs a)
 {
     /* per-lane pathlength accumulators, layer-major so lanes never share a bank word */
     __shared__ float pl_lds[MCML_MAX_LAYERS * REPLAY_BLOCK];
+    __shared__ unsigned long long grid_tiles[REPLAY_BLOCK / 64][TILE_WORDS];
     float* pl_lane = pl_lds + threadIdx.x;
     const int L = a.num_layers;
     const int bnd_min = a.bnd_min < 1 ? 1 : a.bnd_min;
@@ -598,7 +646,11 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
     uint64_t job = 0;
     uint32_t jset = 0, jsds = 1;
     const DevSet* S = a.sets;
-    GridLane gl = {nullptr, nullptr, ~0ull, 0};
+    GridLane gl = {nullptr, nullptr, nullptr, ~0ull, 0, 0, ~0u};
+    if (a.A_rz != nullptr) {
+        gl.tile = grid_tiles[threadIdx.x >> 6];
+        for (int i = lane_id(); i < TILE_WORDS; i += 64) gl.tile[i] = 0ull;
+    }
     uint32_t n_int = 0;
 
     for (;;) {
@@ -614,14 +666,21 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
                 S = a.sets + jset;
                 launch(p, S, a.seed, jset, key & MCML_KEY_K_MASK, a.mult);
                 for (int l = 0; l < L; l++) pl_lane[l * REPLAY_BLOCK] = 0.0f;
-                gl.rl_idx = ~0ull;
+                gl.rl_key = ~0ull;
                 gl.rl_w = 0;
                 if (a.A_rz != nullptr) {
-                    const uint64_t slab = (uint64_t)jset * (uint64_t)a.num_sds + (uint64_t)(jsds - 1);
-                    gl.a_rz = a.A_rz + slab * (uint64_t)a.nz * (uint64_t)a.nr;
-                    gl.a0_z = a.A0_z + slab * (uint64_t)a.nz;
+                    gl.grp = jset * (uint32_t)a.num_sds + (jsds - 1);
+                    gl.a_rz = a.A_rz + (uint64_t)gl.grp * (uint64_t)a.nz * (uint64_t)a.nr;
+                    gl.a0_z = a.A0_z + (uint64_t)gl.grp * (uint64_t)a.nz;
                 }
                 alive = true;
+            }
+            /* wave-uniform: once no live lane uses the tile's group, hand the tile to the
+             * group of the first live lane */
+            if (gl.tile != nullptr && __ballot(alive && gl.grp == gl.tile_grp) == 0 && __ballot(alive) != 0) {
+                flush_tile(gl.tile, gl.tile_grp, a);
+                const uint64_t live = __ballot(alive);
+                gl.tile_grp = (uint32_t)__shfl((int)gl.grp, __ffsll((long long)live) - 1, 64);
             }
         }
         if (__ballot(alive) == 0) break;
@@ -644,45 +703,58 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
         if (st != ST_ALIVE) {
             alive = false;
             pl_lane[(p.pl_layer - 1) * REPLAY_BLOCK] = p.pl_cur;
-            if (gl.a_rz != nullptr && gl.rl_w != 0) atomicAdd(&gl.a_rz[gl.rl_idx], (unsigned long long)gl.rl_w);
+            if (gl.a_rz != nullptr) flush_run(gl, a.nr);
             float* row = a.rows + job * (uint64_t)(L + 2);
             row[0] = (st == ST_DETECTED) ? credited(p, S, (int)jsds) : 0.0f;
             for (int l = 0; l < L; l++) row[1 + l] = pl_lane[l * REPLAY_BLOCK];
             row[L + 1] = (float)p.steps;
         }
     }
+    if (gl.tile != nullptr) flush_tile(gl.tile, gl.tile_grp, a);
 }
 
 /* ----------------------------------------------------------------- sorted records */
-/* Pathlength dump into file order: thread t writes float t of the sorted record image
- * (record t / stride, field t % stride), reading the source row named by `order`; the
- * stores are fully coalesced.  Also writes the sorted weights and public records. */
-extern "C" __global__ void __launch_bounds__(256) mcml_dump(GatherArgs a)
+/* Pathlength dump into file order, in tiles of DUMP_TILE records.  The tile's source
+ * indices are read once into LDS; then thread t copies float t of the tile's record image
+ * (record t / stride, field t % stride, the division by a multiply-shift that is exact in
+ * this range), so both the reads of each source row and the stores are contiguous across
+ * lanes.  Sorted weights and public records are written one record per thread. */
+#define DUMP_TILE 256
+extern "C" __global__ void __launch_bounds__(DUMP_TILE) mcml_dump(GatherArgs a)
 {
-    const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a.stride_out > 0) {
-        const uint64_t so = (uint64_t)a.stride_out;
-        const uint64_t total = a.n * so;
-        for (uint64_t t = t0; t < total; t += gsz) {
-            const uint64_t r = t / so;
-            const uint64_t f = t - r * so;
-            const uint64_t src = a.order != nullptr ? (uint64_t)a.order[r] : r;
-            a.rows_out[t] = a.rows_in[src * (uint64_t)a.stride_in + f];
+    __shared__ uint32_t src_idx[DUMP_TILE];
+    const uint32_t so = (uint32_t)a.stride_out;
+    /* e / so == (e * magic) >> 16 for e < DUMP_TILE * 12 and so <= 12 */
+    const uint32_t magic = so > 0 ? (65536u + so - 1u) / so : 0u;
+    for (uint64_t base = (uint64_t)blockIdx.x * DUMP_TILE; base < a.n; base += (uint64_t)gridDim.x * DUMP_TILE) {
+        const uint64_t left = a.n - base;
+        const uint32_t cnt = left < DUMP_TILE ? (uint32_t)left : DUMP_TILE;
+        const uint64_t i = base + threadIdx.x;
+        if (threadIdx.x < cnt) {
+            const uint32_t src = a.order != nullptr ? a.order[i] : (uint32_t)i;
+            src_idx[threadIdx.x] = src;
+            const float w = a.rows_in[(uint64_t)src * (uint64_t)a.stride_in];
+            a.weight_out[i] = w;
+            if (a.records_out != nullptr) {
+                const uint64_t key = a.sorted_keys[i];
+                mcml_record r;
+                r.key = ((key >> MCML_KEY_SET_SHIFT) << 40) | (key & MCML_KEY_K_MASK);
+                r.sds = (uint32_t)((key >> MCML_KEY_SDS_SHIFT) & 31u);
+                r.weight = w;
+                a.records_out[i] = r;
+            }
         }
-    }
-    for (uint64_t i = t0; i < a.n; i += gsz) {
-        const uint64_t src = a.order != nullptr ? (uint64_t)a.order[i] : i;
-        const float w = a.rows_in[src * (uint64_t)a.stride_in];
-        a.weight_out[i] = w;
-        if (a.records_out != nullptr) {
-            const uint64_t key = a.sorted_keys[i];
-            mcml_record r;
-            r.key = ((key >> MCML_KEY_SET_SHIFT) << 40) | (key & MCML_KEY_K_MASK);
-            r.sds = (uint32_t)((key >> MCML_KEY_SDS_SHIFT) & 31u);
-            r.weight = w;
-            a.records_out[i] = r;
+        if (so > 0) {
+            __syncthreads();
+            float* out = a.rows_out + base * (uint64_t)so;
+            const uint32_t total = cnt * so;
+            for (uint32_t e = threadIdx.x; e < total; e += DUMP_TILE) {
+                const uint32_t rr = (e * magic) >> 16;
+                const uint32_t f = e - rr * so;
+                out[e] = a.rows_in[(uint64_t)src_idx[rr] * (uint64_t)a.stride_in + f];
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -828,7 +900,7 @@ hipError_t mcml_launch_replay(const ReplayArgs& a, int grid, hipStream_t st)
 
 hipError_t mcml_launch_dump(const GatherArgs& a, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL(mcml_dump, dim3(grid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(mcml_dump, dim3(grid), dim3(DUMP_TILE), 0, st, a);
     return hipGetLastError();
 }
 

@@ -225,3 +225,27 @@ def test_cfg3_batch_matches_oracle_on_a_subset(ctx):
         rec, _ = O.first_pass(O.derive(sim), SEED, 0, sim.number_photons, set_index=j)
         mine = photon_order(got[(got["key"] >> 40) == j])
         assert np.array_equal(mine["key"], rec["key"]) and _same_floats(mine["weight"], rec["weight"])
+
+
+@pytest.mark.gpu
+def test_batch_absorbance_grids_match_oracle(ctx):
+    """-A over a batch: each wavefront's LDS tile of the grids' hot corner passes from one
+    (set, SDS) group to the next; every group's u64 grids must still equal the oracle's."""
+    sims = [F.ring_busy(3000), F.ring_busy(5000), F.fiber_busy(4000)]
+    sims[1].layers[0].mua = 0.8
+    sims[2] = F.ring_busy(2000)
+    sims[2].layers[1].mus = 60.0
+    for s in sims:
+        s.grid_nr, s.grid_nz, s.grid_dr, s.grid_dz = 40, 30, 0.01, 0.004
+    res = ctx.run_batch(sims, make_opts(replay=True, pathlength=True, absorbance=True), seed=SEED)
+    for j, sim in enumerate(sims):
+        d = O.derive(sim)
+        for s in range(1, sim.num_sds + 1):
+            arz = np.zeros(30 * 40, np.uint64)
+            a0 = np.zeros(30, np.uint64)
+            keys = np.sort(res.records["key"][(res.records["sds"] == s) & ((res.records["key"] >> 40) == j)])
+            for key in keys:
+                hit, _ = O.replay(d, SEED, int(key & ((1 << 40) - 1)), s, set_index=j, grids=(arz, a0))
+                assert hit == 1
+            assert np.array_equal(res.A_rz[j, s - 1].reshape(-1), arz), (j, s)
+            assert np.array_equal(res.A0_z[j, s - 1], a0), (j, s)
