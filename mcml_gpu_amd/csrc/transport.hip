@@ -139,13 +139,13 @@ __device__ __forceinline__ void spin(Photon& p)
     if (p.hg_c == 0.0f) { /* g == 0 */
         cost = 2.0f * mcml_uniform(&p.rng) - 1.0f;
     } else {
-        const float temp = p.hg_a / (p.hg_b + p.hg_c * mcml_uniform(&p.rng));
-        cost = (p.hg_d - temp * temp) * p.hg_inv2g;
+        const float temp = p.hg_a / fmaf(p.hg_c, mcml_uniform(&p.rng), p.hg_b);
+        cost = fmaf(-temp, temp, p.hg_d) * p.hg_inv2g;
     }
-    const float a = 1.0f - cost * cost;
+    const float a = fmaf(-cost, cost, 1.0f);
     float sinp, cosp;
     mcml_sincos_turns(mcml_uniform(&p.rng), &sinp, &cosp);
-    const float b = 1.0f - p.dz * p.dz;
+    const float b = fmaf(-p.dz, p.dz, 1.0f);
     if (b == 0.0f) {
         const float sint = sqrtf(a);
         p.dx = sint * cosp;
@@ -154,13 +154,13 @@ __device__ __forceinline__ void spin(Photon& p)
     } else {
         const float P = sqrtf(a * b);
         const float Q = P / b;
-        const float cxp = Q * (p.dx * p.dz * cosp - p.dy * sinp) + p.dx * cost;
-        const float cyp = Q * (p.dy * p.dz * cosp + p.dx * sinp) + p.dy * cost;
-        p.dz = -P * cosp + p.dz * cost;
+        const float cxp = fmaf(Q, fmaf(p.dx * p.dz, cosp, -(p.dy * sinp)), p.dx * cost);
+        const float cyp = fmaf(Q, fmaf(p.dy * p.dz, cosp, p.dx * sinp), p.dy * cost);
+        p.dz = fmaf(-P, cosp, p.dz * cost);
         p.dx = cxp;
         p.dy = cyp;
     }
-    const float d = p.dx * p.dx + p.dy * p.dy + p.dz * p.dz - 1.0f;
+    const float d = fmaf(p.dx, p.dx, fmaf(p.dy, p.dy, fmaf(p.dz, p.dz, -1.0f)));
     const float r = fmaf(d, fmaf(d, 0.375f, -0.5f), 1.0f);
     p.dx = p.dx * r;
     p.dy = p.dy * r;
@@ -264,17 +264,17 @@ __device__ __forceinline__ bool hop(Photon& p, float& s, int& nl, float* pl_lane
 {
     s = -mcml_log_unit(mcml_uniform(&p.rng)) * p.mutr;
     nl = p.layer;
-    if (p.z + s * p.dz < p.zmin) {
+    if (fmaf(s, p.dz, p.z) < p.zmin) {
         nl--;
         s = (p.zmin - p.z) / p.dz;
     }
-    if (p.z + s * p.dz > p.zmax) {
+    if (fmaf(s, p.dz, p.z) > p.zmax) {
         nl++;
         s = (p.zmax - p.z) / p.dz;
     }
-    p.x += p.dx * s;
-    p.y += p.dy * s;
-    p.z += p.dz * s;
+    p.x = fmaf(p.dx, s, p.x);
+    p.y = fmaf(p.dy, s, p.y);
+    p.z = fmaf(p.dz, s, p.z);
     p.steps++;
     if (REPLAY) p.pl_cur += s; /* transport.cu:529: pathlength of the layer the hop started in */
     if (p.z > p.zmax) p.z = p.zmax;

@@ -165,13 +165,13 @@ float oracle_spin(oracle_photon* p, float g)
         cost = 2.0f * urand(p) - 1.0f;
     } else {
         float inv2g = 1.0f / (2.0f * g);
-        float temp = (1.0f - g * g) / (1.0f - g + 2.0f * g * urand(p));
-        cost = (1.0f + g * g - temp * temp) * inv2g;
+        float temp = (1.0f - g * g) / fmaf(2.0f * g, urand(p), 1.0f - g);
+        cost = fmaf(-temp, temp, 1.0f + g * g) * inv2g;
     }
-    float a = 1.0f - cost * cost;
+    float a = fmaf(-cost, cost, 1.0f);
     float sinp, cosp;
     mcml_sincos_turns(urand(p), &sinp, &cosp);
-    float b = 1.0f - p->dz * p->dz;
+    float b = fmaf(-p->dz, p->dz, 1.0f);
     if (b == 0.0f) { /* travelling along z */
         float sint = sqrtf(a);
         p->dx = sint * cosp;
@@ -180,13 +180,13 @@ float oracle_spin(oracle_photon* p, float g)
     } else {
         float P = sqrtf(a * b);
         float Q = P / b;
-        float cxp = Q * (p->dx * p->dz * cosp - p->dy * sinp) + p->dx * cost;
-        float cyp = Q * (p->dy * p->dz * cosp + p->dx * sinp) + p->dy * cost;
-        p->dz = -P * cosp + p->dz * cost;
+        float cxp = fmaf(Q, fmaf(p->dx * p->dz, cosp, -(p->dy * sinp)), p->dx * cost);
+        float cyp = fmaf(Q, fmaf(p->dy * p->dz, cosp, p->dx * sinp), p->dy * cost);
+        p->dz = fmaf(-P, cosp, p->dz * cost);
         p->dx = cxp;
         p->dy = cyp;
     }
-    float d = p->dx * p->dx + p->dy * p->dy + p->dz * p->dz - 1.0f;
+    float d = fmaf(p->dx, p->dx, fmaf(p->dy, p->dy, fmaf(p->dz, p->dz, -1.0f)));
     float r = fmaf(d, fmaf(d, 0.375f, -0.5f), 1.0f);
     p->dx = p->dx * r;
     p->dy = p->dy * r;
@@ -322,17 +322,17 @@ uint32_t oracle_photon_life(const mcml_derived* d, oracle_photon* p, int target,
         const mcml_layer_derived* Ly = &d->layer[p->layer];
         float s = -mcml_log_unit(urand(p)) * Ly->mutr; /* transport.cu:373 */
         int new_layer = p->layer;
-        if (p->z + s * p->dz < Ly->z_min) { /* transport.cu:379-386 */
+        if (fmaf(s, p->dz, p->z) < Ly->z_min) { /* transport.cu:379-386 */
             new_layer--;
             s = (Ly->z_min - p->z) / p->dz;
         }
-        if (p->z + s * p->dz > Ly->z_max) {
+        if (fmaf(s, p->dz, p->z) > Ly->z_max) {
             new_layer++;
             s = (Ly->z_max - p->z) / p->dz;
         }
-        p->x += p->dx * s;
-        p->y += p->dy * s;
-        p->z += p->dz * s;
+        p->x = fmaf(p->dx, s, p->x);
+        p->y = fmaf(p->dy, s, p->y);
+        p->z = fmaf(p->dz, s, p->z);
         p->steps++;
         if (replay) pl[p->layer - 1] += s; /* transport.cu:529 */
         if (p->z > Ly->z_max) p->z = Ly->z_max;

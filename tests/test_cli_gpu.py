@@ -1,0 +1,86 @@
+"""The MCML_GPU executable end to end on the GPU: two input lines in one batched launch, -R -P
+-AP -A -B with a fixed seed.  Every file must hold what the library returns for the same
+batch and seed (GPU)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import mcml_gpu_amd as M
+
+SEED = 4242
+LAYERS = [  # h, mua, mus, n, g per layer; two simulations
+    [(0.1, 0.3, 60.0, 1.4, 0.8), (0.3, 0.05, 120.0, 1.37, 0.9), (0.0, 0.2, 80.0, 1.45, 0.85)],
+    [(0.2, 0.6, 90.0, 1.38, 0.85), (0.2, 0.1, 100.0, 1.4, 0.9), (0.0, 0.3, 70.0, 1.42, 0.8)],
+]
+
+
+def write_inputs(tmp_path, photons):
+    cfg = {"number_simulation": 2, "number_photons": photons, "number_layers": 3, "detector_reflectance": 0.35,
+           "upper_n": 1.457, "lower_n": 1.457, "source_probe_oblique": 1, "detector_probe_oblique": 1,
+           "grid_nr": 40, "grid_nz": 30, "grid_dr": 0.02, "grid_dz": 0.01,
+           "probes": {"source": {"NA": 0.37, "radius": 0.075, "angle": 8.0}, "num_SDS": 3,
+                      "detectors": [{"pos": 0.25, "NA": 0.5, "radius": 0.15, "angle": 10.0},
+                                    {"pos": 0.5, "NA": 0.6, "radius": 0.2, "angle": 0.0},
+                                    {"pos": 0.9, "NA": 0.6, "radius": 0.3, "angle": 25.0}]}}
+    js = tmp_path / "sim_set.json"
+    js.write_text(json.dumps(cfg))
+    lines = []
+    for layers in LAYERS:
+        vals = []
+        for i, (h, mua, mus, n, g) in enumerate(layers):
+            if i < len(layers) - 1:
+                vals.append(h)
+            vals += [mua, mus, n, g]
+        lines.append(" ".join(repr(v) for v in vals))
+    txt = tmp_path / "input.txt"
+    txt.write_text("\n".join(lines) + "\n")
+    return str(js), str(txt)
+
+
+@pytest.mark.gpu
+def test_cli_outputs_equal_library_results(tmp_path):
+    photons = 200000
+    js, txt = write_inputs(tmp_path, photons)
+    r = subprocess.run([M.EXE_PATH, js, txt, "output.txt", "-R", "-P", "-AP", "-A", "-B", "-S", str(SEED)],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    sims = M.read_input(js, txt)
+    assert len(sims) == 2
+    with M.Context(0) as ctx:
+        res = ctx.run_batch(sims, M.make_opts(replay=True, pathlength=True, avg_pathlength=True, absorbance=True,
+                                              binary=True), seed=SEED)
+    S, L = 3, 3
+    # output.txt: one line per simulation, replayed reflectance per SDS / (0xFFFFFFFF N)
+    rows = [[float(v) for v in line.split()] for line in (tmp_path / "output.txt").read_text().splitlines()]
+    assert len(rows) == 2
+    for j in range(2):
+        want = res.replay_reflectance[j] / (4294967295.0 * photons)
+        assert np.allclose(rows[j], want, rtol=1e-5, atol=0)
+    # pathlength_SDS_#.bin: binary files are overwritten, so they hold the last simulation
+    for s in range(1, S + 1):
+        got = np.fromfile(tmp_path / ("pathlength_SDS_%d.bin" % s), dtype="<f4").reshape(-1, L + 2)
+        assert np.array_equal(got, res.pathlength_records(1, s))
+        assert len(got) == res.detected[1, s - 1] > 0
+    # average_pathlength.txt: appended, one line per simulation
+    avg = [[float(v) for v in line.split()] for line in (tmp_path / "average_pathlength.txt").read_text().splitlines()]
+    assert len(avg) == 2 and all(len(a) == S * L for a in avg)
+    assert all(np.all(np.isfinite(a)) and min(a) >= 0 for a in avg)
+    # absorbance maps: last simulation, FP32 binary, z-major
+    for s in range(1, S + 1):
+        a = np.fromfile(tmp_path / ("A_rz_SDS_%d.bin" % s), dtype="<f4")
+        assert a.size == 30 * 40 and np.all(a >= 0)
+        nz_cells = res.A_rz[1, s - 1].reshape(-1) > 0
+        assert np.array_equal(a > 0, nz_cells)
+    # summary.json: appended, one JSON object per simulation
+    text, objs, dec, i = (tmp_path / "summary.json").read_text(), [], json.JSONDecoder(), 0
+    while i < len(text):
+        if text[i].isspace():
+            i += 1
+            continue
+        obj, i = dec.raw_decode(text, i)
+        objs.append(obj)
+    assert len(objs) == 2
