@@ -2,7 +2,8 @@
 """Benchmark of the photon-transport hot path on one MI355X.
 
 Workload: the 5-layer slab (cfg1): h 0.3/0.5/0.2/0.3 cm + semi-infinite, 1e8 photons,
-replay with per-photon pathlength dump (-R -P -B).  One step = one full hot-path pass on
+replay with per-photon pathlength dump (-R -P -B).  --config cfg0/cfg2 or the reference's
+ring-detection examples (head_model, hw8a) run the same measurement on those inputs.  One step = one full hot-path pass on
 device: persistent first-pass transport of all photons, replay of every detected photon,
 and the pathlength dump; inputs are resident and nothing is written to disk.
 
@@ -31,6 +32,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 FLOP_INTERACTION = 62  # step 2, boundary test 2, move 6, drop 3, HG 11, sin 3, psi 3, sqrt 3, rotate 20, renorm 9
 FLOP_BOUNDARY = 44     # step, clip, move, Fresnel + Snell on transmission
 PEAK_FP32_TFLOPS = 157.3
+EXAMPLES = {"head_model": "tests/golden/head_model", "hw8a": "tests/golden/HW8_A"}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -85,7 +87,11 @@ def main():
         return bench_batch(args)
     if args.config == "cfg4":
         return bench_surrogate(args)
-    sim = M.load_config(args.config)
+    if args.config in EXAMPLES:  # the reference's own example inputs (ring detection)
+        d = os.path.join(ROOT, EXAMPLES[args.config])
+        sim = M.read_input(os.path.join(d, "sim_setup.json"), os.path.join(d, "GPUMC_input.txt"))[0]
+    else:
+        sim = M.load_config(args.config)
     sim.number_photons = int(args.photons)
     opts = M.make_opts(replay=True, pathlength=True, binary=True, block_size=args.block,
                        chunk=args.chunk, refill_min=args.refill, blocks_per_cu=args.bpc,
@@ -126,8 +132,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "%s: 5-layer slab (h 0.3/0.5/0.2/0.3 cm + semi-inf, g 0.9, n 1.4), "
-                        "%.0e photons, replay + pathlength dump (-R -P -B)" % (args.config, n),
+            "workload": "%s: %d-layer slab, %d SDS (%s detection), %.0e photons, replay + pathlength dump "
+                        "(-R -P -B)" % (args.config, sim.number_layers, sim.num_sds,
+                                        "fiber" if sim.detector_probe_oblique else "ring", n),
             "photons": n,
             "layers": sim.number_layers,
             "sds": sim.num_sds,

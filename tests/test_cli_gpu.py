@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import mcml_gpu_amd as M
+from mcml_gpu_amd import loaders
 
 SEED = 4242
 LAYERS = [  # h, mua, mus, n, g per layer; two simulations
@@ -76,11 +77,12 @@ def test_cli_outputs_equal_library_results(tmp_path):
         nz_cells = res.A_rz[1, s - 1].reshape(-1) > 0
         assert np.array_equal(a > 0, nz_cells)
     # summary.json: appended, one JSON object per simulation
-    text, objs, dec, i = (tmp_path / "summary.json").read_text(), [], json.JSONDecoder(), 0
-    while i < len(text):
-        if text[i].isspace():
-            i += 1
-            continue
-        obj, i = dec.raw_decode(text, i)
-        objs.append(obj)
+    objs = loaders.read_summaries(tmp_path / "summary.json")
     assert len(objs) == 2
+    assert [o["SDS_detected_number"] for o in objs] == [list(map(int, res.detected[j])) for j in range(2)]
+    # the NumPy loaders read the same files
+    for s in range(1, S + 1):
+        got = loaders.load_binary_pathlength(tmp_path / "summary.json", s, tmp_path / ("pathlength_SDS_%d.bin" % s))
+        assert np.array_equal(got, res.pathlength_records(1, s))
+        amap = loaders.load_A_arr(tmp_path, s, nz=30, nr=40)
+        assert amap.shape == (30, 40) and np.all(amap >= 0)
