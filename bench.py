@@ -33,6 +33,12 @@ FLOP_INTERACTION = 62  # step 2, boundary test 2, move 6, drop 3, HG 11, sin 3, 
 FLOP_BOUNDARY = 44     # step, clip, move, Fresnel + Snell on transmission
 PEAK_FP32_TFLOPS = 157.3
 EXAMPLES = {"head_model": "tests/golden/head_model", "hw8a": "tests/golden/HW8_A"}
+# HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each scaled by what the
+# counter reads for a plain 1 GiB device copy (profiles/transport_cfg1_1e8.md,
+# profiles/surrogate_fused_cfg4.md): transport 3.13e5 B read + 2.632e7 B written per 1e8
+# photons (ticket atomics and counters); fused inference 8.76e6 + 1.10e6 B per 1e5 rows.
+TRANSPORT_TRAFFIC_PER_PHOTON = (3.133e5 + 2.632e7) / 1e8
+MLP_TRAFFIC_PER_SAMPLE = (8.756e6 + 1.100e6) / 1e5
 HBM_PEAK_GBS = 8000.0
 
 
@@ -151,13 +157,13 @@ def main():
         "detected": detected,
         "roofline": {
             "bound": "valu",
-            "kernel": "mcml_first_pass",
+            "kernel": "mcml_first_pass_pl",  # -R without -A records pathlengths in the first pass
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "flop_per_photon": flop_per_photon,
-            "traffic": None,
+            "traffic": TRANSPORT_TRAFFIC_PER_PHOTON * n,
         },
         "dump": {"bytes": dump_bytes,
                  "GB_per_s": (dump_bytes / (np.mean(dump_ms) * 1e-3) / 1e9) if np.mean(dump_ms) > 0 else None},
@@ -217,9 +223,9 @@ def bench_batch(args):
         "kernel_photons_per_s": total / (k_ms * 1e-3), "first_pass_kernel_ms": k_ms,
         "first_pass_wall_ms": float(np.mean(walls_fp)), "launches": res.first_pass_launches,
         "detected": int(res.detected.sum()), "steps_per_photon": res.steps / total,
-        "roofline": {"bound": "valu", "kernel": "mcml_first_pass", "achieved": achieved,
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": None},
+        "roofline": {"bound": "valu", "kernel": "mcml_first_pass_pl" if args.pathlength else "mcml_first_pass",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRANSPORT_TRAFFIC_PER_PHOTON * total},
     }
     if args.pathlength:
         n_rec = int(res.detected.sum())
@@ -309,7 +315,7 @@ def bench_surrogate(args):
         "heldout_s": t_ref,
         "roofline": {"bound": "mfma", "kernel": "mcml_mlp_fused", "achieved": achieved,
                      "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS,
-                     "flop_per_sample": MLP_FLOP_PER_SAMPLE, "traffic": None},
+                     "flop_per_sample": MLP_FLOP_PER_SAMPLE, "traffic": MLP_TRAFFIC_PER_SAMPLE * args.infer_batch},
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
