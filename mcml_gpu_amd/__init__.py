@@ -100,20 +100,49 @@ def make_opts(replay=False, absorbance=False, pathlength=False, avg_pathlength=F
                     int(skip_records))
 
 
-class RunResult:
-    """Copies of an mcml_result's arrays as numpy arrays."""
+class _ResultOwner:
+    """Owns one library-allocated mcml_result and frees it (mcml_result_free) once the last
+    numpy view of its buffers is gone."""
 
-    def __init__(self, r: abi.Result):
+    def __init__(self, lib, r):
+        self.lib, self.r = lib, r
+
+    def __del__(self):
+        try:
+            self.lib.mcml_result_free(C.byref(self.r))
+        except Exception:
+            pass
+
+
+def _view(owner, ptr, count, dtype):
+    """numpy view (no copy) of `count` elements of `dtype` at ptr that keeps `owner` alive."""
+    dtype = np.dtype(dtype)
+    if count == 0 or not ptr:
+        return np.zeros(0, dtype)
+    buf = (C.c_uint8 * (count * dtype.itemsize)).from_address(C.cast(ptr, C.c_void_p).value)
+    buf._owner = owner
+    return np.frombuffer(buf, dtype=dtype, count=count)
+
+
+class RunResult:
+    """An mcml_result as numpy arrays.  The large buffers (records, pathlength rows,
+    absorbance grids) are views of the library's memory, released when the last of them is
+    dropped; the per-SDS sums are small copies."""
+
+    def __init__(self, r: abi.Result, lib=None):
+        owner = _ResultOwner(lib, r) if lib is not None else None
         self.nsets, self.num_sds, self.num_layers = r.nsets, r.num_sds, r.num_layers
         g = r.nsets * r.num_sds
         self.reflectance = np.ctypeslib.as_array(r.reflectance, (g,)).copy().reshape(r.nsets, r.num_sds)
         self.detected = np.ctypeslib.as_array(r.detected, (g,)).copy().reshape(r.nsets, r.num_sds)
+
+        def big(ptr, count, dtype):
+            if owner is not None:
+                return _view(owner, ptr, count, dtype)
+            return _view(None, ptr, count, dtype).copy()
+
         n = r.n_records if r.records else 0
-        if n:
-            raw = np.ctypeslib.as_array(C.cast(r.records, C.POINTER(C.c_uint8)), (n * 16,)).copy()
-            self.records = raw.view(np.dtype(abi.RECORD_DTYPE))
-        else:
-            self.records = np.zeros(0, dtype=abi.RECORD_DTYPE)
+        self.records = big(r.records, n, abi.RECORD_DTYPE)
         self.replayed = bool(r.replayed)
         if self.replayed:
             self.replay_reflectance = np.ctypeslib.as_array(r.replay_reflectance, (g,)).copy().reshape(
@@ -121,13 +150,11 @@ class RunResult:
             self.pl_offset = np.ctypeslib.as_array(r.pl_offset, (g + 1,)).copy()
             nrec = int(self.pl_offset[-1])
             w = r.num_layers + 2
-            self.pathlength = (np.ctypeslib.as_array(r.pathlength, (nrec * w,)).copy().reshape(nrec, w)
-                               if nrec else np.zeros((0, w), np.float32))
+            self.pathlength = big(r.pathlength, nrec * w, np.float32).reshape(nrec, w)
             if r.A_rz:
-                self.A_rz = np.ctypeslib.as_array(r.A_rz, (g * r.grid_nz * r.grid_nr,)).copy().reshape(
+                self.A_rz = big(r.A_rz, g * r.grid_nz * r.grid_nr, np.uint64).reshape(
                     r.nsets, r.num_sds, r.grid_nz, r.grid_nr)
-                self.A0_z = np.ctypeslib.as_array(r.A0_z, (g * r.grid_nz,)).copy().reshape(
-                    r.nsets, r.num_sds, r.grid_nz)
+                self.A0_z = big(r.A0_z, g * r.grid_nz, np.uint64).reshape(r.nsets, r.num_sds, r.grid_nz)
             else:
                 self.A_rz = self.A0_z = None
         for f in ("photons", "steps", "steps_boundary", "steps_interaction", "ms_first_pass_kernel",
@@ -170,21 +197,22 @@ class Context:
         self.lib.mcml_result_free(C.byref(r))
 
     def run(self, sim, opts=None, seed=1):
-        r = self.run_raw(sim, opts, seed)
+        return self._wrap(self.run_raw(sim, opts, seed))
+
+    def _wrap(self, r):
+        """RunResult owning r (freed when its last array is dropped)."""
         try:
-            return RunResult(r)
-        finally:
-            self.free_raw(r)
+            return RunResult(r, lib=self.lib)
+        except BaseException:
+            self.lib.mcml_result_free(C.byref(r))
+            raise
 
     def run_batch(self, sims, opts=None, seed=1):
         arr = (abi.Sim * len(sims))(*sims)
         r = abi.Result()
         o = opts if opts is not None else make_opts()
         self._check(self.lib.mcml_run_batch(self.h, arr, len(sims), C.byref(o), seed, C.byref(r)))
-        try:
-            return RunResult(r)
-        finally:
-            self.lib.mcml_result_free(C.byref(r))
+        return self._wrap(r)
 
     def autotune(self, sims, trial_photons=10_000_000, opts=None):
         """Launch shape picked by mcml_autotune for these sets (returns a filled Opts)."""
