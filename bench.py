@@ -99,9 +99,13 @@ def main():
     else:
         sim = M.load_config(args.config)
     sim.number_photons = int(args.photons)
-    opts = M.make_opts(replay=True, pathlength=True, binary=True, block_size=args.block,
+    # cfg2 is the absorbance-map workload (-R -A -B: the first pass, then a replay of every
+    # detected photon into the A_rz/A0_z grids); the others dump pathlengths (-R -P -B)
+    absorb = args.config == "cfg2"
+    opts = M.make_opts(replay=True, pathlength=not absorb, absorbance=absorb, binary=True, block_size=args.block,
                        chunk=args.chunk, refill_min=args.refill, blocks_per_cu=args.bpc,
                        boundary_min=args.bnd)
+    flags = "-R -A -B" if absorb else "-R -P -B"
     ctx = M.Context(0)
     for _ in range(args.warmup):
         ctx.run(sim, opts, seed=args.seed)
@@ -138,9 +142,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "%s: %d-layer slab, %d SDS (%s detection), %.0e photons, replay + pathlength dump "
-                        "(-R -P -B)" % (args.config, sim.number_layers, sim.num_sds,
-                                        "fiber" if sim.detector_probe_oblique else "ring", n),
+            "workload": "%s: %d-layer slab, %d SDS (%s detection), %.0e photons, %s" % (
+                args.config, sim.number_layers, sim.num_sds, "fiber" if sim.detector_probe_oblique else "ring", n,
+                "replay into %d x %d absorbance grids (%s)" % (sim.grid_nz, sim.grid_nr, flags) if absorb
+                else "replay + pathlength dump (%s)" % flags),
             "photons": n,
             "layers": sim.number_layers,
             "sds": sim.num_sds,
@@ -157,7 +162,7 @@ def main():
         "detected": detected,
         "roofline": {
             "bound": "valu",
-            "kernel": "mcml_first_pass_pl",  # -R without -A records pathlengths in the first pass
+            "kernel": "mcml_first_pass" if absorb else "mcml_first_pass_pl",  # -R -P: inline pathlengths
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",

@@ -51,12 +51,14 @@ MCML_HD uint32_t mcml_mwc_next(mcml_mwc* r)
     return (uint32_t)r->s;
 }
 
-/* Uniform in (0, 1]: the top 24 bits plus one, times 2^-24.  Exact in binary32, never 0,
- * so -log(u) is finite (the same half-open interval as the reference's generator). */
+/* Uniform in (0, 1]: the top 23 bits k become 1 - k 2^-23, computed exactly as 2 minus the
+ * float 1.k in [1, 2) built from the bits (three integer/float operations, no conversion).
+ * Never 0, so -log(u) is finite (the same half-open interval as the reference's
+ * generator); the smallest value is 2^-23. */
 MCML_HD float mcml_uniform(mcml_mwc* r)
 {
     uint32_t x = mcml_mwc_next(r);
-    return (float)((x >> 8) + 1u) * 5.9604644775390625e-08f; /* 2^-24 */
+    return 2.0f - mcml_u2f(0x3F800000u | (x >> 9));
 }
 
 MCML_HD uint64_t mcml_mix64(uint64_t z)
@@ -126,7 +128,7 @@ MCML_HD float mcml_log(float x)
     return fmaf(fe, MCML_LN2_HI, fmaf(fe, MCML_LN2_LO, lm));
 }
 
-/* log(u) for the generator's outputs u in [2^-24, 1] (never subnormal): the same
+/* log(u) for the generator's outputs u in [2^-23, 1] (never subnormal): the same
  * arithmetic as mcml_log without the subnormal rescaling. */
 MCML_HD float mcml_log_unit(float x)
 {

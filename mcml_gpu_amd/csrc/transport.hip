@@ -135,25 +135,33 @@ __device__ __forceinline__ void launch(Photon& p, const DevSet* S, uint64_t seed
  * rsqrt about 1 for the renormalisation (the norm never drifts more than a few ulp). */
 __device__ __forceinline__ void spin(Photon& p)
 {
-    float cost;
-    if (p.hg_c == 0.0f) { /* g == 0 */
-        cost = 2.0f * mcml_uniform(&p.rng) - 1.0f;
-    } else {
-        const float temp = p.hg_a / fmaf(p.hg_c, mcml_uniform(&p.rng), p.hg_b);
-        cost = fmaf(-temp, temp, p.hg_d) * p.hg_inv2g;
-    }
-    const float a = fmaf(-cost, cost, 1.0f);
+    const float u = mcml_uniform(&p.rng);
+    const bool iso = p.hg_c == 0.0f; /* g == 0 */
+    const float D = iso ? 1.0f : fmaf(p.hg_c, u, p.hg_b); /* HG denominator 1 - g + 2gu */
     float sinp, cosp;
     mcml_sincos_turns(mcml_uniform(&p.rng), &sinp, &cosp);
     const float b = fmaf(-p.dz, p.dz, 1.0f);
     if (b == 0.0f) {
-        const float sint = sqrtf(a);
+        float cost = 2.0f * u - 1.0f;
+        if (!iso) {
+            const float temp = p.hg_a / D;
+            cost = fmaf(-temp, temp, p.hg_d) * p.hg_inv2g;
+        }
+        const float sint = sqrtf(fmaf(-cost, cost, 1.0f));
         p.dx = sint * cosp;
         p.dy = sint * sinp;
         p.dz = copysignf(cost, p.dz * cost);
     } else {
+        /* one division serves the HG inversion (1/D = b r) and the rotation (1/b = D r) */
+        const float r = 1.0f / (D * b);
+        float cost = 2.0f * u - 1.0f;
+        if (!iso) {
+            const float temp = p.hg_a * (b * r);
+            cost = fmaf(-temp, temp, p.hg_d) * p.hg_inv2g;
+        }
+        const float a = fmaf(-cost, cost, 1.0f);
         const float P = sqrtf(a * b);
-        const float Q = P / b;
+        const float Q = P * (D * r);
         const float cxp = fmaf(Q, fmaf(p.dx * p.dz, cosp, -(p.dy * sinp)), p.dx * cost);
         const float cyp = fmaf(Q, fmaf(p.dy * p.dz, cosp, p.dx * sinp), p.dy * cost);
         p.dz = fmaf(-P, cosp, p.dz * cost);

@@ -155,31 +155,41 @@ void oracle_launch(const mcml_derived* d, uint64_t seed, uint32_t set, uint64_t 
  *   cos(theta) = (1 + g^2 - t^2) * (1 / 2g)              (reference divides by 2g)
  *   P = sqrt((1 - cos^2)(1 - dz^2)), Q = P / (1 - dz^2)    (= sin(theta) * sqrt(1-dz^2) and
  *                                                           sin(theta) / sqrt(1-dz^2))
+ *   with one reciprocal r = 1 / (D (1 - dz^2)), D = 1 - g + 2gu, for both divisions:
+ *   t = (1 - g^2) / D = (1 - g^2)((1 - dz^2) r) and Q = P (D r)
  *   renormalisation by 1 - d/2 + 3d^2/8, d = |u|^2 - 1     (second-order rsqrt about 1;
  *                                                           |d| stays at a few ulp)
  * Returns cos(theta) for the tests. */
 float oracle_spin(oracle_photon* p, float g)
 {
-    float cost;
-    if (g == 0.0f) {
-        cost = 2.0f * urand(p) - 1.0f;
-    } else {
-        float inv2g = 1.0f / (2.0f * g);
-        float temp = (1.0f - g * g) / fmaf(2.0f * g, urand(p), 1.0f - g);
-        cost = fmaf(-temp, temp, 1.0f + g * g) * inv2g;
-    }
-    float a = fmaf(-cost, cost, 1.0f);
+    const float u = urand(p);
+    const int iso = g == 0.0f;
+    const float D = iso ? 1.0f : fmaf(2.0f * g, u, 1.0f - g); /* HG denominator 1 - g + 2gu */
     float sinp, cosp;
     mcml_sincos_turns(urand(p), &sinp, &cosp);
     float b = fmaf(-p->dz, p->dz, 1.0f);
+    float cost;
     if (b == 0.0f) { /* travelling along z */
-        float sint = sqrtf(a);
+        cost = 2.0f * u - 1.0f;
+        if (!iso) {
+            float temp = (1.0f - g * g) / D;
+            cost = fmaf(-temp, temp, 1.0f + g * g) * (1.0f / (2.0f * g));
+        }
+        float sint = sqrtf(fmaf(-cost, cost, 1.0f));
         p->dx = sint * cosp;
         p->dy = sint * sinp;
         p->dz = copysignf(cost, p->dz * cost);
     } else {
+        /* one division serves the HG inversion (1/D = b r) and the rotation (1/b = D r) */
+        float r = 1.0f / (D * b);
+        cost = 2.0f * u - 1.0f;
+        if (!iso) {
+            float temp = (1.0f - g * g) * (b * r);
+            cost = fmaf(-temp, temp, 1.0f + g * g) * (1.0f / (2.0f * g));
+        }
+        float a = fmaf(-cost, cost, 1.0f);
         float P = sqrtf(a * b);
-        float Q = P / b;
+        float Q = P * (D * r);
         float cxp = fmaf(Q, fmaf(p->dx * p->dz, cosp, -(p->dy * sinp)), p->dx * cost);
         float cyp = fmaf(Q, fmaf(p->dy * p->dz, cosp, p->dx * sinp), p->dy * cost);
         p->dz = fmaf(-P, cosp, p->dz * cost);

@@ -14,9 +14,10 @@ def ulp_err(y32, ref64):
 
 
 def test_log_all_generator_outputs():
-    # every value the generator can produce: k * 2^-24, k = 1 .. 2^24
-    k = np.arange(1, 2 ** 24 + 1, dtype=np.float64)
-    u = (k * 2.0 ** -24).astype(np.float32)
+    # every value the generator can produce: 1 - k * 2^-23, k = 2^23 - 1 .. 0 (ascending)
+    k = np.arange(2 ** 23 - 1, -1, -1, dtype=np.float64)
+    u = (1.0 - k * 2.0 ** -23).astype(np.float32)
+    assert np.array_equal(u.astype(np.float64), 1.0 - k * 2.0 ** -23)  # exact in binary32
     y = np.zeros_like(u)
     O.lib().oracle_log_unit_n(u, y, len(u))
     ref = np.log(u.astype(np.float64))

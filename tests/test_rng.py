@@ -22,7 +22,7 @@ def test_recurrence_matches_bigint_restatement():
             t = mult * (state & 0xFFFFFFFF) + (state >> 32)  # x' = a x + c, carry = high word
             state = t
             x = t & 0xFFFFFFFF
-            exp.append(((x >> 8) + 1) / 2.0 ** 24)
+            exp.append(1.0 - (x >> 9) / 2.0 ** 23)  # 2 - 1.k, exact
         assert np.array_equal(got, np.array(exp, np.float32))
 
 
