@@ -153,7 +153,7 @@ __device__ __forceinline__ void spin(Photon& p)
         p.dz = copysignf(cost, p.dz * cost);
     } else {
         /* one division serves the HG inversion (1/D = b r) and the rotation (1/b = D r) */
-        const float r = 1.0f / (D * b);
+        const float r = mcml_rcp(D * b);
         float cost = 2.0f * u - 1.0f;
         if (!iso) {
             const float temp = p.hg_a * (b * r);

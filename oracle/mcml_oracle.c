@@ -181,7 +181,7 @@ float oracle_spin(oracle_photon* p, float g)
         p->dz = copysignf(cost, p->dz * cost);
     } else {
         /* one division serves the HG inversion (1/D = b r) and the rotation (1/b = D r) */
-        float r = 1.0f / (D * b);
+        float r = mcml_rcp(D * b);
         cost = 2.0f * u - 1.0f;
         if (!iso) {
             float temp = (1.0f - g * g) * (b * r);
@@ -522,6 +522,11 @@ void oracle_sincos_turns_n(const float* t, float* s, float* c, int64_t n)
 {
     for (int64_t i = 0; i < n; i++) mcml_sincos_turns(t[i], &s[i], &c[i]);
 }
+void oracle_rcp_n(const float* x, float* y, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) y[i] = mcml_rcp(x[i]);
+}
+
 void oracle_acos_n(const float* x, float* y, int64_t n)
 {
     for (int64_t i = 0; i < n; i++) y[i] = mcml_acos(x[i]);

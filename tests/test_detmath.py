@@ -73,3 +73,14 @@ def test_acos():
     yb = np.zeros_like(bad)
     O.lib().oracle_acos_n(bad, yb, len(bad))
     assert np.isnan(yb).all()
+
+
+def test_rcp_within_half_ulp():
+    """mcml_rcp (seed + three Newton steps) against float64 1/x over the ranges the spin uses
+    (1 - dz^2 down to 2^-23 times the HG denominator) and across binades."""
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(1e-8, 2.5, 2000000), 10.0 ** rng.uniform(-30, 30, 500000),
+                        2.0 ** np.arange(-60, 60), np.nextafter(2.0 ** np.arange(-60, 60), 0)]).astype(np.float32)
+    y = np.zeros_like(x)
+    O.lib().oracle_rcp_n(x, y, len(x))
+    assert ulp_err(y, 1.0 / x.astype(np.float64)).max() < 0.52
