@@ -159,8 +159,8 @@ __device__ __forceinline__ void spin(Photon& p)
             const float temp = p.hg_a * (b * r);
             cost = fmaf(-temp, temp, p.hg_d) * p.hg_inv2g;
         }
-        const float a = fmaf(-cost, cost, 1.0f);
-        const float P = sqrtf(a * b);
+        const float ab = fmaf(-cost, cost, 1.0f) * b;
+        const float P = ab * mcml_rsqrt(ab);
         const float Q = P * (D * r);
         const float cxp = fmaf(Q, fmaf(p.dx * p.dz, cosp, -(p.dy * sinp)), p.dx * cost);
         const float cyp = fmaf(Q, fmaf(p.dy * p.dz, cosp, p.dx * sinp), p.dy * cost);

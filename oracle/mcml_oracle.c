@@ -156,7 +156,7 @@ void oracle_launch(const mcml_derived* d, uint64_t seed, uint32_t set, uint64_t 
  *   P = sqrt((1 - cos^2)(1 - dz^2)), Q = P / (1 - dz^2)    (= sin(theta) * sqrt(1-dz^2) and
  *                                                           sin(theta) / sqrt(1-dz^2))
  *   with one reciprocal r = 1 / (D (1 - dz^2)), D = 1 - g + 2gu, for both divisions:
- *   t = (1 - g^2) / D = (1 - g^2)((1 - dz^2) r) and Q = P (D r)
+ *   t = (1 - g^2) / D = (1 - g^2)((1 - dz^2) r) and Q = P (D r); P = x rsqrt(x)
  *   renormalisation by 1 - d/2 + 3d^2/8, d = |u|^2 - 1     (second-order rsqrt about 1;
  *                                                           |d| stays at a few ulp)
  * Returns cos(theta) for the tests. */
@@ -187,8 +187,8 @@ float oracle_spin(oracle_photon* p, float g)
             float temp = (1.0f - g * g) * (b * r);
             cost = fmaf(-temp, temp, 1.0f + g * g) * (1.0f / (2.0f * g));
         }
-        float a = fmaf(-cost, cost, 1.0f);
-        float P = sqrtf(a * b);
+        float ab = fmaf(-cost, cost, 1.0f) * b;
+        float P = ab * mcml_rsqrt(ab);
         float Q = P * (D * r);
         float cxp = fmaf(Q, fmaf(p->dx * p->dz, cosp, -(p->dy * sinp)), p->dx * cost);
         float cyp = fmaf(Q, fmaf(p->dy * p->dz, cosp, p->dx * sinp), p->dy * cost);
@@ -525,6 +525,11 @@ void oracle_sincos_turns_n(const float* t, float* s, float* c, int64_t n)
 void oracle_rcp_n(const float* x, float* y, int64_t n)
 {
     for (int64_t i = 0; i < n; i++) y[i] = mcml_rcp(x[i]);
+}
+
+void oracle_rsqrt_n(const float* x, float* y, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) y[i] = mcml_rsqrt(x[i]);
 }
 
 void oracle_acos_n(const float* x, float* y, int64_t n)
