@@ -202,6 +202,7 @@ def bench_batch(args):
     ctx = M.Context(0)
     tuned = ctx.autotune(sims, trial_photons=2e7)
     opts = M.make_opts(block_size=tuned.block_size, chunk=tuned.chunk, refill_min=tuned.refill_min,
+                       boundary_min=tuned.boundary_min,
                        use_graph=True, skip_records=not args.pathlength, replay=args.pathlength,
                        pathlength=args.pathlength, binary=args.pathlength)
     for _ in range(args.warmup):
@@ -224,7 +225,7 @@ def bench_batch(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "cfg3: 512 three-layer sets, budgets log-uniform 1e5-1e7, one persistent launch",
                    "photons": total, "sets": len(sims), "block": res.block_size, "chunk": res.chunk,
-                   "refill_min": res.refill_min, "graph": True},
+                   "refill_min": res.refill_min, "boundary_min": res.boundary_min, "graph": True},
         "kernel_photons_per_s": total / (k_ms * 1e-3), "first_pass_kernel_ms": k_ms,
         "first_pass_wall_ms": float(np.mean(walls_fp)), "launches": res.first_pass_launches,
         "detected": int(res.detected.sum()), "steps_per_photon": res.steps / total,

@@ -213,9 +213,10 @@ int mcml_run_batch(mcml_ctx* ctx, const mcml_sim* sims, int nsets, const mcml_op
 
 void mcml_result_free(mcml_result* res);
 
-/* Pick the launch shape (block_size, chunk, refill_min) for these parameter sets: times the
- * first pass of a reduced copy of the batch (about `trial_photons` photons in total) for
- * each candidate shape and writes the fastest into *opts (other fields untouched).  No
+/* Pick the launch shape (block_size, chunk, refill_min, boundary_min) for these parameter
+ * sets: times the first pass of a reduced copy of the batch (about `trial_photons` photons in
+ * total) for each candidate shape and writes the fastest into *opts (other fields
+ * untouched; a boundary_min already set in *opts is kept).  No
  * counterpart in the reference, which sizes its grid from compute-capability tables
  * (io.cpp:471-493). */
 int mcml_autotune(mcml_ctx* ctx, const mcml_sim* sims, int nsets, uint64_t trial_photons, mcml_opts* opts);

@@ -34,7 +34,7 @@ constexpr float kPI = 3.141592654f; /* header.h:40 */
 /* launch defaults (results never depend on them: each photon's stream is fixed) */
 constexpr int kDefaultChunk = 128;
 constexpr int kDefaultRefill = 2;
-constexpr int kDefaultBoundary = 1;
+constexpr int kDefaultBoundary = 4;
 constexpr size_t kLdsTableMax = 16384;
 constexpr int kCtrlWords = 8;
 constexpr int kCtrlFirstPass = 6; /* words the first pass resets and reads back */
@@ -754,20 +754,23 @@ extern "C" int mcml_autotune(mcml_ctx* c, const mcml_sim* sims, int nsets, uint6
     if (rc) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
     const int blocks[] = {128, 256};
-    const int refills[] = {1, 2, 4};
+    /* (refill threshold, boundary batch) pairs: idle lanes to wait for before a refill, and
+     * lanes at a boundary to wait for before resolving boundaries together */
+    const int rb[][2] = {{1, 1}, {2, 1}, {2, 4}, {4, 4}};
     const int chunks[] = {64, 128, 256};
     double best = 1e300;
     mcml_opts pick = *opts;
     bool warmed = false;
     for (int b : blocks)
-        for (int r : refills)
+        for (const auto& p : rb)
             for (int ch : chunks) {
+                const int r = p[0];
                 mcml_opts o;
                 memset(&o, 0, sizeof(o));
                 o.block_size = b;
                 o.refill_min = r;
                 o.chunk = ch;
-                o.boundary_min = opts->boundary_min;
+                o.boundary_min = opts->boundary_min > 0 ? opts->boundary_min : p[1];
                 for (int rep = warmed ? 0 : -1; rep < 1; rep++) { /* one untimed warm-up run */
                     mcml_result res;
                     memset(&res, 0, sizeof(res));
@@ -779,6 +782,7 @@ extern "C" int mcml_autotune(mcml_ctx* c, const mcml_sim* sims, int nsets, uint6
                         pick.block_size = b;
                         pick.refill_min = r;
                         pick.chunk = ch;
+                        pick.boundary_min = o.boundary_min;
                     }
                 }
                 warmed = true;

@@ -207,9 +207,10 @@ def test_autotune_returns_a_valid_shape(ctx):
     sims, _, _ = batch.cfg3_sets(16, seed=3, budget_lo=1e3, budget_hi=1e4)
     o = ctx.autotune(sims, trial_photons=200000)
     assert o.block_size in (128, 256) and o.refill_min in (1, 2, 4) and o.chunk in (64, 128, 256)
+    assert o.boundary_min in (1, 4)
     a = ctx.run_batch(sims, make_opts(), seed=SEED)
     b = ctx.run_batch(sims, make_opts(block_size=o.block_size, chunk=o.chunk, refill_min=o.refill_min,
-                                      use_graph=True), seed=SEED)
+                                      boundary_min=o.boundary_min, use_graph=True), seed=SEED)
     assert np.array_equal(a.records, b.records)
 
 
