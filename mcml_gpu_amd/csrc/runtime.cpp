@@ -583,6 +583,13 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
     std::vector<mcml_derived> der;
     int rc = validate_batch(c, sims, nsets, der);
     if (rc) return rc;
+    if (o.replay && o.absorbance)
+        for (int j = 1; j < nsets; j++)
+            if (der[(size_t)j].grid_nr != der[0].grid_nr || der[(size_t)j].grid_nz != der[0].grid_nz ||
+                der[(size_t)j].grid_dr != der[0].grid_dr || der[(size_t)j].grid_dz != der[0].grid_dz) {
+                set_err(c, "set %d: all sets of a batch must share the absorbance grid (-A)", j);
+                return MCML_ERR_ARG;
+            }
     HIP_TRY(c, hipSetDevice(c->device));
     const int S = der[0].num_sds, L = der[0].num_layers;
     const int nr = der[0].grid_nr, nz = der[0].grid_nz;

@@ -250,3 +250,18 @@ def test_batch_absorbance_grids_match_oracle(ctx):
                 assert hit == 1
             assert np.array_equal(res.A_rz[j, s - 1].reshape(-1), arz), (j, s)
             assert np.array_equal(res.A0_z[j, s - 1], a0), (j, s)
+
+
+@pytest.mark.gpu
+def test_batch_rejects_mixed_absorbance_grids_and_accepts_empty_sets(ctx):
+    from mcml_gpu_amd import MCMLError
+    a, b = F.ring_busy(100), F.ring_busy(100)
+    b.grid_nr = a.grid_nr + 1
+    with pytest.raises(MCMLError):
+        ctx.run_batch([a, b], make_opts(replay=True, absorbance=True), seed=SEED)
+    # without -A the grids do not matter; sets with no photons are fine
+    c = F.ring_busy(0)
+    res = ctx.run_batch([a, b, c], make_opts(replay=True, pathlength=True), seed=SEED)
+    assert res.photons == 200 and res.detected[2].sum() == 0
+    res = ctx.run(F.ring_busy(0), make_opts(), seed=SEED)
+    assert res.photons == 0 and len(res.records) == 0
