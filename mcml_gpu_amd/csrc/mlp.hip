@@ -48,9 +48,10 @@ __device__ __forceinline__ float normal_cdf(float z)
     return 0.5f + 0.5f * copysignf(erf_abs, z);
 }
 
-/* GELU(z) = z Phi(z) with the same erf approximation, rearranged for fewer operations:
- * z (1 + sign(z) erf(|z|/sqrt 2)) / 2 = max(z, 0) - |z| (poly(t) / 2) e^{-z^2/2}, where
- * t = 1 / (1 + 0.3275911 |z| / sqrt 2); the 1/2 and 1/sqrt 2 are folded into constants. */
+/* GELU(z) = z Phi(z) with the same erf approximation (A&S 7.1.26), rearranged for fewer
+ * operations: z (1 + sign(z) erf(|z|/sqrt 2)) / 2 = max(z, 0) - |z| (poly(t) / 2) e^{-z^2/2},
+ * t = 1 / (1 + 0.3275911 |z| / sqrt 2); the 1/2 and 1/sqrt 2 are folded into the constants
+ * and e^{-z^2/2} = 2^(z (z (-log2(e) / 2))). */
 __device__ __forceinline__ float gelu(float z)
 {
     const float az = fabsf(z);
@@ -60,7 +61,7 @@ __device__ __forceinline__ float gelu(float z)
     p = fmaf(p, t, 0.5f * -0.284496736f);
     p = fmaf(p, t, 0.5f * 0.254829592f);
     p *= t;
-    const float e = __expf(z * (-0.5f * z));
+    const float e = __builtin_amdgcn_exp2f(z * (z * -0.72134752044448170f));
     return fmaf(-az, p * e, fmaxf(z, 0.0f));
 }
 
@@ -327,23 +328,24 @@ extern "C" __global__ void __launch_bounds__(MLP_FUSED_WAVES * 64) mcml_mlp_fuse
     __syncthreads();
     stage<FW_KT * 4 * 16>(wlds, pk + a.w_off[a.hidden] / 8);
     __syncthreads();
+    const float* bl = a.bias + a.b_off[a.hidden];
+    const float4 bq = *reinterpret_cast<const float4*>(bl + 4 * q); /* padded to 16 outputs */
     f32x4 acc[FW_MB];
 #pragma unroll
-    for (int mb = 0; mb < FW_MB; mb++) acc[mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int mb = 0; mb < FW_MB; mb++) acc[mb] = f32x4{bq.x, bq.y, bq.z, bq.w};
 #pragma unroll
     for (int t = 0; t < FW_KT; t++) {
         const bf16x8 w = as_bf16x8(wlds[(t * 4 + q) * 16 + r]);
 #pragma unroll
         for (int mb = 0; mb < FW_MB; mb++) acc[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, act[t][mb], acc[mb], 0, 0, 0);
     }
-    const float* bl = a.bias + a.b_off[a.hidden];
 #pragma unroll
     for (int mb = 0; mb < FW_MB; mb++) {
         const int m = row0 + 16 * mb + r;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int n = 4 * q + e;
-            if (m < a.B && n < a.outs) a.y[(int64_t)m * a.outs + n] = acc[mb][e] + bl[n];
+            if (m < a.B && n < a.outs) a.y[(int64_t)m * a.outs + n] = acc[mb][e];
         }
     }
 }
