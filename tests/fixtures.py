@@ -87,3 +87,16 @@ def ring_busy(photons):
                                      (0.7, 0.5, 0.2, 0.0)])
     cfg = base_cfg(photons, 2, 1.457, 1.3, 0, 0, prb, refl=0.1)
     return abi.make_sim(cfg, [(0.15, 0.4, 70.0, 1.4, 0.85), (0.0, 0.1, 110.0, 1.38, 0.9)])
+
+
+def max_stack(photons, ring=False):
+    """The largest problem the formats allow: 10 layers (header.h:38) with alternating
+    indices, and 30 detectors (header.h:44) spread from 0.05 to 1.5 cm, fibre or ring mode."""
+    dets = [(0.05 + 0.05 * i, 0.6, 0.04, 5.0 * (i % 3) if not ring else 0.0) for i in range(30)]
+    cfg = base_cfg(photons, 10, 1.45, 1.33, 0 if ring else 1, 0 if ring else 1, probes((0.3, 0.05, 0.0), dets),
+                   refl=0.2)
+    layers = []
+    for i in range(10):
+        h = 0.0 if i == 9 else 0.04 + 0.02 * (i % 4)
+        layers.append((h, 0.05 + 0.03 * i, 40.0 + 9.0 * i, 1.33 + 0.015 * (i % 5), 0.7 + 0.025 * (i % 8)))
+    return abi.make_sim(cfg, layers)

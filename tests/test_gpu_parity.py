@@ -28,6 +28,8 @@ CASES = {
     "head_model": lambda: F.example("head_model", 20000),
     "cfg0": lambda: F.config("cfg0", 20000),
     "cfg1": lambda: F.config("cfg1", 4000),
+    "max_stack_fiber": lambda: F.max_stack(20000),
+    "max_stack_ring": lambda: F.max_stack(20000, ring=True),
 }
 
 
@@ -76,7 +78,8 @@ def test_first_pass_matches_oracle(ctx, name):
                                                                rel=1e-12, abs=0)
 
 
-REPLAY_CASES = ["fiber_busy", "ring_busy", "HW8_A", "head_model", "hemisphere_g09_n14"]
+REPLAY_CASES = ["fiber_busy", "ring_busy", "HW8_A", "head_model", "hemisphere_g09_n14", "max_stack_fiber",
+                "max_stack_ring"]
 
 
 @pytest.mark.gpu
