@@ -12,6 +12,10 @@ roofline = the first-pass transport kernel, FP32 VALU bound: algorithmic FLOP pe
 (62 per interaction step + 44 per boundary step, counted on the same run) divided by the
 kernel's average duration from HIP events on the library's stream.
 cpu_baseline = the CPU restatement of the same loop (oracle, OpenMP) on a bounded sample.
+reference_shape = one run of the same photon step in the reference's launch structure
+(32-thread workgroups, one photon per thread, 50 000-step relaunches with per-launch host
+read-back and reduction, then a separate replay), timed the same way, as the stand-in for
+the reference's own photons/s; `speedup` = value / its photons/s.
 
 --config cfg3: the 512-set batch in one persistent launch (photons/s).
 --config cfg4: the reflectance surrogate (R evaluations/s of the trained network at its
@@ -51,6 +55,9 @@ def main():
     ap.add_argument("--config", default="cfg1")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-refshape", action="store_true", help="skip the reference-launch-shape run")
+    ap.add_argument("--refshape-block", type=int, default=32,
+                    help="workgroup size of the reference-launch-shape run (the reference uses 32)")
     ap.add_argument("--cpu-photons", type=float, default=0, help="CPU baseline sample size")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
@@ -173,6 +180,29 @@ def main():
         "dump": {"bytes": dump_bytes,
                  "GB_per_s": (dump_bytes / (np.mean(dump_ms) * 1e-3) / 1e9) if np.mean(dump_ms) > 0 else None},
     }
+
+    if not args.no_refshape:
+        # the same photon step in the reference's launch structure (one photon per thread,
+        # 32-thread workgroups, 50 000-step relaunches, per-launch host read-back and
+        # reduction of per-thread detection buffers), then the replay as a separate pass:
+        # the stated stand-in for the reference's own photons/s, which cannot run here
+        ropts = M.make_opts(replay=True, pathlength=not absorb, absorbance=absorb, binary=True,
+                            reference_shape=True, block_size=args.refshape_block)
+        ts = time.perf_counter()
+        rr = ctx.run(sim, ropts, seed=args.seed)
+        rwall = time.perf_counter() - ts
+        line["reference_shape"] = {
+            "value": n / rwall, "unit": "photons/s",
+            "kernel_photons_per_s": n / (rr.ms_first_pass_kernel * 1e-3),
+            "first_pass_kernel_ms": rr.ms_first_pass_kernel,
+            "first_pass_wall_ms": rr.ms_first_pass_wall,
+            "replay_kernel_ms": rr.ms_replay_kernel,
+            "wall_ms": rwall * 1e3, "launches": rr.first_pass_launches,
+            "block": rr.block_size, "threads": rr.block_size * rr.grid_size,
+            "speedup": value / (n / rwall),
+            "kernel_speedup": rr.ms_first_pass_kernel / k_ms,
+            "detected": int(rr.detected.sum()),
+        }
 
     if not args.no_cpu:
         import oracle_ffi as O

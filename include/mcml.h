@@ -131,11 +131,18 @@ typedef struct {
     int32_t binary;          /* -B  (writers only) */
     int32_t block_size;      /* threads per workgroup; 0 = tuned default */
     int32_t blocks_per_cu;   /* resident workgroups per CU; 0 = occupancy maximum */
-    int32_t chunk;           /* photon ids a wavefront claims per queue access; 0 = tuned */
+    int32_t chunk;           /* photon ids a wavefront claims per queue access; 0 = tuned.
+                                With reference_shape: steps per launch (0 = 50 000) */
     int32_t refill_min;      /* refill when at least this many lanes are idle; 0 = tuned */
     int32_t boundary_min;    /* resolve boundary hits when at least this many lanes wait on
                                 one; 0 = tuned (results never depend on the launch knobs) */
-    int32_t reference_shape; /* reserved, must be 0 */
+    int32_t reference_shape; /* 1 = run the first pass in the reference's launch structure
+                                (transport.cu:76-147): one photon per thread in workgroups of
+                                block_size threads (0 = 32), 50 000-step relaunches, per-launch
+                                host read-back and reduction of per-thread detection buffers;
+                                replay follows as a separate pass even without -A.  One set
+                                per run.  Same records and sums as 0; kept as a measured
+                                comparison, it is several times slower */
     int32_t use_graph;       /* 1 = run the first pass through a cached HIP graph (reset
                                 kernel + transport kernel + counter read-back), updating
                                 only its kernel arguments between calls */

@@ -97,6 +97,38 @@ struct ReplayArgs {
     float dr, dz;
 };
 
+/* Reference launch shape (transport.cu:57-147, 341-471): one photon per thread, at most
+ * MCML_REF_STEPS steps per launch, photon state in global memory between launches.
+ * RefFibers is one thread's detection buffer for one launch (Fibers, header.h:127-133):
+ * the photon key stands in for the reference's saved curand state. */
+#define MCML_REF_STEPS 50000 /* NUMSTEPS_GPU, header.h:39 */
+#define MCML_REF_SLOTS 20    /* SDS_detected_temp_size, header.h:61 */
+struct RefFibers {
+    int32_t count;
+    int32_t sds[MCML_REF_SLOTS];
+    float w[MCML_REF_SLOTS];
+    uint64_t key[MCML_REF_SLOTS];
+};
+
+struct RefShapeArgs {
+    const DevSet* set;         /* the one parameter set of the run */
+    uint64_t total;            /* photons */
+    uint64_t threads;          /* T photon slots; photon k < T starts in slot k */
+    uint64_t seed;
+    const uint32_t* mult;
+    void* state;               /* [T] slot state between launches (mcml_refshape_slot_bytes each) */
+    uint32_t* active;          /* [T] 1 while the slot still has a photon */
+    RefFibers* fibers;         /* [T] this launch's detections */
+    unsigned long long* terminated; /* photons ended so far; photon T + t is the t-th relaunch */
+    uint64_t* spill_key;       /* detections beyond a full RefFibers (the reference drops them) */
+    float* spill_w;
+    unsigned long long* n_spill;
+    uint64_t spill_cap;
+    unsigned long long* counters; /* [CNT_N] */
+    int32_t steps;             /* step budget of one launch */
+    int32_t pad;
+};
+
 /* Records in sorted order: gather rows, build the public records, reduce per (set, SDS). */
 struct GatherArgs {
     const uint64_t* sorted_keys;

@@ -30,5 +30,9 @@ hipError_t mcml_sort_keys(void* temp, size_t& temp_bytes, const uint64_t* keys_i
                           uint32_t* idx_out, uint64_t n, hipStream_t st);
 hipError_t mcml_occupancy_replay(int* blocks_per_cu);
 int mcml_replay_block();
+/* reference launch shape: bytes of one slot's saved state; init and one step-budget launch */
+size_t mcml_refshape_slot_bytes();
+hipError_t mcml_launch_refshape_init(const RefShapeArgs& a, int block, hipStream_t st);
+hipError_t mcml_launch_refshape_pass(const RefShapeArgs& a, int block, hipStream_t st);
 
 #endif

@@ -67,6 +67,7 @@ struct mcml_ctx {
     DevBuf offsets, sums;             /* per (set, SDS) group: start offsets, FP64 sums */
     DevBuf rrows, arz, a0z;           /* replay rows and absorbance grids */
     DevBuf wrows, woff, wold, wnew, wout; /* White Monte Carlo inputs and output */
+    DevBuf ref_state, ref_active, ref_fibers, ref_spill_key, ref_spill_w; /* reference launch shape */
 };
 
 namespace {
@@ -337,7 +338,8 @@ extern "C" void mcml_destroy(mcml_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->sets,  &c->set_begin, &c->rec_key, &c->rec_row, &c->skey,  &c->sidx, &c->stemp,
                       &c->rows,  &c->weights,   &c->precs,   &c->offsets, &c->sums,  &c->rrows, &c->arz,
-                      &c->a0z,   &c->wrows,     &c->woff,    &c->wold,    &c->wnew,  &c->wout};
+                      &c->a0z,   &c->wrows,     &c->woff,    &c->wold,    &c->wnew,  &c->wout,
+                      &c->ref_state, &c->ref_active, &c->ref_fibers, &c->ref_spill_key, &c->ref_spill_w};
     for (DevBuf* b : bufs) release(*b);
     mcml_graph_release(c->graph);
     if (c->d_mult) (void)hipFree(c->d_mult);
@@ -506,6 +508,168 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
     return MCML_OK;
 }
 
+/* First pass in the reference's launch structure (transport.cu:76-147, io.cpp:487-526):
+ * T = maxThreadsPerMultiProcessor x CUs threads in workgroups of `block` (the reference
+ * picks 32), one photon per thread, relaunched every MCML_REF_STEPS steps (o.chunk
+ * when set, so that tests can force many relaunches) until no thread
+ * is active; after each launch the host clears and uploads the per-thread detection
+ * buffers, reads back the active flags and the buffers and reduces them (fiber.cu:3-13,
+ * transport.cu:299-311).  The records end in c->rec_key / c->rec_row like first_pass's,
+ * and host_count holds the per-detector counts of the host reduction. */
+int first_pass_refshape(mcml_ctx* c, const mcml_derived& d, const mcml_opts& o, uint64_t seed, mcml_result* out,
+                        uint64_t& n_rec, std::vector<uint64_t>& host_count)
+{
+    const int S = d.num_sds;
+    DevSet hs;
+    fill_devset(d, hs);
+    hs.ticket_begin = 0;
+    const uint64_t total = d.number_photons;
+    int rc;
+    if ((rc = ensure(c, c->sets, sizeof(DevSet)))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->sets.p, &hs, sizeof(DevSet), hipMemcpyHostToDevice, c->stream));
+    const int block = o.block_size > 0 ? o.block_size : 32;
+    if (block % 32 != 0 || block > 256) {
+        set_err(c, "block_size must be a multiple of 32 and at most 256 in the reference launch shape");
+        return MCML_ERR_ARG;
+    }
+    int max_mp = 0;
+    HIP_TRY(c, hipDeviceGetAttribute(&max_mp, hipDeviceAttributeMaxThreadsPerMultiProcessor, c->device));
+    const uint64_t blocks = (uint64_t)std::max(1, max_mp / block) * (uint64_t)c->cu_count;
+    const uint64_t T = blocks * (uint64_t)block;
+    if ((rc = ensure(c, c->ref_state, mcml_refshape_slot_bytes() * (size_t)T)) ||
+        (rc = ensure(c, c->ref_active, sizeof(uint32_t) * (size_t)T)) ||
+        (rc = ensure(c, c->ref_fibers, sizeof(RefFibers) * (size_t)T)))
+        return rc;
+    /* detections beyond a thread's MCML_REF_SLOTS in one launch: kept (the reference drops
+     * them, transport.cu:899-904), so the records equal the persistent kernel's */
+    uint64_t spill_cap = std::min<uint64_t>(total * (uint64_t)S, std::max<uint64_t>(1ull << 16, total / 64 + 1024));
+    std::vector<RefFibers> hf((size_t)T); /* pageable, as HostMem.f (transport.cu:1059) */
+    std::vector<uint32_t> hact((size_t)T);
+    std::vector<uint64_t> hkey;
+    std::vector<float> hw;
+    std::vector<double> hsum((size_t)S);
+    double ms_kernel = 0;
+    int launches = 0;
+    uint64_t n_spill = 0;
+    for (;;) {
+        if ((rc = ensure(c, c->ref_spill_key, sizeof(uint64_t) * (size_t)spill_cap)) ||
+            (rc = ensure(c, c->ref_spill_w, sizeof(float) * (size_t)spill_cap)))
+            return rc;
+        RefShapeArgs a;
+        memset(&a, 0, sizeof(a));
+        a.set = as<DevSet>(c->sets);
+        a.total = total;
+        a.threads = T;
+        a.seed = seed;
+        a.mult = c->d_mult;
+        a.state = c->ref_state.p;
+        a.active = as<uint32_t>(c->ref_active);
+        a.fibers = as<RefFibers>(c->ref_fibers);
+        a.terminated = c->ctrl + 0;
+        a.n_spill = c->ctrl + 1;
+        a.counters = c->ctrl + 2;
+        a.spill_key = as<uint64_t>(c->ref_spill_key);
+        a.spill_w = as<float>(c->ref_spill_w);
+        a.spill_cap = spill_cap;
+        a.steps = o.chunk > 0 ? o.chunk : MCML_REF_STEPS;
+        hkey.clear();
+        hw.clear();
+        std::fill(hsum.begin(), hsum.end(), 0.0);
+        host_count.assign((size_t)S, 0);
+        ms_kernel = 0;
+        launches = 0;
+        HIP_TRY(c, mcml_launch_reset(c->ctrl, kCtrlFirstPass, c->stream));
+        HIP_TRY(c, mcml_launch_refshape_init(a, block, c->stream));
+        uint64_t active_total = 1;
+        while (active_total > 0) {
+            memset(hf.data(), 0, sizeof(RefFibers) * hf.size()); /* fiber_initialization */
+            HIP_TRY(c, hipMemcpyAsync(a.fibers, hf.data(), sizeof(RefFibers) * hf.size(), hipMemcpyHostToDevice,
+                                      c->stream));
+            HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+            HIP_TRY(c, mcml_launch_refshape_pass(a, block, c->stream));
+            HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(hact.data(), a.active, sizeof(uint32_t) * hact.size(), hipMemcpyDeviceToHost,
+                                      c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            ms_kernel += ms_between(c->ev0, c->ev1);
+            launches++;
+            active_total = 0;
+            for (uint32_t v : hact) active_total += v;
+            HIP_TRY(c, hipMemcpyAsync(hf.data(), a.fibers, sizeof(RefFibers) * hf.size(), hipMemcpyDeviceToHost,
+                                      c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            for (const RefFibers& f : hf) /* calculate_reflectance */
+                for (int k = 0; k < f.count; k++) {
+                    const int s = f.sds[k];
+                    if (s < 1 || s > S) {
+                        set_err(c, "reference launch shape: detector index %d out of range", s);
+                        return MCML_ERR_DEVICE;
+                    }
+                    hsum[(size_t)s - 1] += (double)f.w[k];
+                    host_count[(size_t)s - 1]++;
+                    hkey.push_back(f.key[k]);
+                    hw.push_back(f.w[k]);
+                }
+        }
+        HIP_TRY(c, hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(unsigned long long) * kCtrlFirstPass,
+                                  hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        n_spill = c->h_ctrl[1];
+        if (n_spill <= spill_cap) break;
+        spill_cap = n_spill + n_spill / 8 + 1024; /* rerun: each photon's stream is fixed */
+    }
+    if (c->h_ctrl[0] != total) {
+        set_err(c, "reference launch shape ended %llu photons, expected %llu", (unsigned long long)c->h_ctrl[0],
+                (unsigned long long)total);
+        return MCML_ERR_DEVICE;
+    }
+    n_rec = n_spill + (uint64_t)hkey.size();
+    if (n_rec >= (1ull << 32)) {
+        set_err(c, "more than 2^32 detection records");
+        return MCML_ERR_LIMIT;
+    }
+    if ((rc = ensure(c, c->rec_key, sizeof(uint64_t) * (size_t)n_rec)) ||
+        (rc = ensure(c, c->rec_row, sizeof(float) * (size_t)n_rec)))
+        return rc;
+    if (n_spill) {
+        HIP_TRY(c, hipMemcpyAsync(c->rec_key.p, c->ref_spill_key.p, sizeof(uint64_t) * (size_t)n_spill,
+                                  hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->rec_row.p, c->ref_spill_w.p, sizeof(float) * (size_t)n_spill,
+                                  hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (!hkey.empty()) {
+        HIP_TRY(c, hipMemcpyAsync(as<uint64_t>(c->rec_key) + n_spill, hkey.data(), sizeof(uint64_t) * hkey.size(),
+                                  hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(as<float>(c->rec_row) + n_spill, hw.data(), sizeof(float) * hw.size(),
+                                  hipMemcpyHostToDevice, c->stream));
+    }
+    if (n_spill) { /* spilled records count toward the per-detector totals as well */
+        std::vector<uint64_t> sk((size_t)n_spill);
+        HIP_TRY(c, hipMemcpyAsync(sk.data(), c->ref_spill_key.p, sizeof(uint64_t) * (size_t)n_spill,
+                                  hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (uint64_t key : sk) {
+            const uint64_t s = (key >> MCML_KEY_SDS_SHIFT) & 31u;
+            if (s >= 1 && s <= (uint64_t)S) host_count[(size_t)s - 1]++;
+        }
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    out->photons = std::min<uint64_t>(T, total) + c->h_ctrl[2 + CNT_PHOTONS];
+    out->steps = c->h_ctrl[2 + CNT_STEPS];
+    out->steps_boundary = c->h_ctrl[2 + CNT_BOUNDARY];
+    out->steps_interaction = c->h_ctrl[2 + CNT_INTERACTION];
+    out->ms_first_pass_kernel = ms_kernel;
+    out->first_pass_launches = launches;
+    out->block_size = block;
+    out->grid_size = (int)blocks;
+    if (out->photons != total) {
+        set_err(c, "reference launch shape launched %llu photons, expected %llu", (unsigned long long)out->photons,
+                (unsigned long long)total);
+        return MCML_ERR_DEVICE;
+    }
+    return MCML_OK;
+}
+
 /* Sort the records by (set, SDS, photon) on the device. */
 int sort_records(mcml_ctx* c, uint64_t n)
 {
@@ -583,6 +747,10 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
     std::vector<mcml_derived> der;
     int rc = validate_batch(c, sims, nsets, der);
     if (rc) return rc;
+    if (o.reference_shape && nsets != 1) {
+        set_err(c, "the reference launch shape runs one parameter set at a time");
+        return MCML_ERR_ARG;
+    }
     if (o.replay && o.absorbance)
         for (int j = 1; j < nsets; j++)
             if (der[(size_t)j].grid_nr != der[0].grid_nr || der[(size_t)j].grid_nz != der[0].grid_nz ||
@@ -605,20 +773,31 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
         mcml_result_free(out);
         return MCML_ERR_NOMEM;
     }
-    /* replay without absorbance maps: record the pathlengths in the first pass itself */
-    const bool inline_pl = o.replay && !o.absorbance;
+    /* replay without absorbance maps: record the pathlengths in the first pass itself
+     * (the reference launch shape replays, as the reference does) */
+    const bool inline_pl = o.replay && !o.absorbance && !o.reference_shape;
     const int stride = inline_pl ? L + 2 : 1;
     const double t0 = now_ms();
     uint64_t n = 0;
-    std::vector<uint64_t> off;
+    std::vector<uint64_t> off, host_count;
     double dump_ms = 0;
-    if ((rc = first_pass(c, der, o, seed, stride, out, n)) || (rc = sort_records(c, n)) ||
+    rc = o.reference_shape ? first_pass_refshape(c, der[0], o, seed, out, n, host_count)
+                           : first_pass(c, der, o, seed, stride, out, n);
+    if (rc || (rc = sort_records(c, n)) ||
         (rc = gather_and_reduce(c, n, nsets, S, as<float>(c->rec_row), stride, as<uint32_t>(c->sidx),
                                 inline_pl ? L + 2 : 0, !o.skip_records, out->reflectance, off, &dump_ms))) {
         mcml_result_free(out);
         return rc;
     }
     for (size_t g = 0; g < groups; g++) out->detected[g] = off[g + 1] - off[g];
+    if (o.reference_shape)
+        for (size_t g = 0; g < groups; g++)
+            if (out->detected[g] != host_count[g]) {
+                set_err(c, "reference launch shape: SDS %d host reduction counted %llu records, the device %llu",
+                        (int)g + 1, (unsigned long long)host_count[g], (unsigned long long)out->detected[g]);
+                mcml_result_free(out);
+                return MCML_ERR_DEVICE;
+            }
     out->n_records = n;
     if (!o.skip_records) {
         out->records = (mcml_record*)malloc(sizeof(mcml_record) * (size_t)std::max<uint64_t>(n, 1));
@@ -647,15 +826,19 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
                 HIP_TRY(c, hipMemcpy(out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2),
                                      hipMemcpyDeviceToHost));
         } else { /* absorbance maps need the true replay (transport.cu:160-222) */
-            const size_t rz_cells = groups * (size_t)nz * (size_t)nr, z_cells = groups * (size_t)nz;
+            const bool grids = o.absorbance != 0;
+            const size_t rz_cells = grids ? groups * (size_t)nz * (size_t)nr : 0;
+            const size_t z_cells = grids ? groups * (size_t)nz : 0;
             if ((rc = ensure(c, c->rrows, sizeof(float) * (size_t)n * (size_t)(L + 2))) ||
                 (rc = ensure(c, c->arz, sizeof(unsigned long long) * rz_cells)) ||
                 (rc = ensure(c, c->a0z, sizeof(unsigned long long) * z_cells))) {
                 mcml_result_free(out);
                 return rc;
             }
-            HIP_TRY(c, hipMemsetAsync(c->arz.p, 0, sizeof(unsigned long long) * rz_cells, c->stream));
-            HIP_TRY(c, hipMemsetAsync(c->a0z.p, 0, sizeof(unsigned long long) * z_cells, c->stream));
+            if (grids) {
+                HIP_TRY(c, hipMemsetAsync(c->arz.p, 0, sizeof(unsigned long long) * rz_cells, c->stream));
+                HIP_TRY(c, hipMemsetAsync(c->a0z.p, 0, sizeof(unsigned long long) * z_cells, c->stream));
+            }
             if (n) {
                 HIP_TRY(c, hipMemsetAsync(c->ctrl + kCtrlReplayQueue, 0, sizeof(unsigned long long), c->stream));
                 ReplayArgs a;
@@ -670,8 +853,8 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
                 a.num_layers = L;
                 a.bnd_min = o.boundary_min > 0 ? std::min(o.boundary_min, 64) : kDefaultBoundary;
                 a.rows = as<float>(c->rrows);
-                a.A_rz = as<unsigned long long>(c->arz);
-                a.A0_z = as<unsigned long long>(c->a0z);
+                a.A_rz = grids ? as<unsigned long long>(c->arz) : nullptr;
+                a.A0_z = grids ? as<unsigned long long>(c->a0z) : nullptr;
                 a.num_sds = S;
                 a.nr = nr;
                 a.nz = nz;
@@ -698,14 +881,16 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
             if (n)
                 HIP_TRY(c, hipMemcpy(out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2),
                                      hipMemcpyDeviceToHost));
-            out->A_rz = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(rz_cells, 1));
-            out->A0_z = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(z_cells, 1));
-            if (!out->A_rz || !out->A0_z) {
-                mcml_result_free(out);
-                return MCML_ERR_NOMEM;
+            if (grids) {
+                out->A_rz = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(rz_cells, 1));
+                out->A0_z = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(z_cells, 1));
+                if (!out->A_rz || !out->A0_z) {
+                    mcml_result_free(out);
+                    return MCML_ERR_NOMEM;
+                }
+                HIP_TRY(c, hipMemcpy(out->A_rz, c->arz.p, sizeof(uint64_t) * rz_cells, hipMemcpyDeviceToHost));
+                HIP_TRY(c, hipMemcpy(out->A0_z, c->a0z.p, sizeof(uint64_t) * z_cells, hipMemcpyDeviceToHost));
             }
-            HIP_TRY(c, hipMemcpy(out->A_rz, c->arz.p, sizeof(uint64_t) * rz_cells, hipMemcpyDeviceToHost));
-            HIP_TRY(c, hipMemcpy(out->A0_z, c->a0z.p, sizeof(uint64_t) * z_cells, hipMemcpyDeviceToHost));
         }
         out->replayed = 1;
         out->ms_replay_wall = now_ms() - t1;

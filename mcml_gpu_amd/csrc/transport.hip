@@ -13,6 +13,9 @@
  *                    and the absorbance grids (transport.cu:473-622, 941-1000).
  *   mcml_dump        packs the replay outputs into the on-disk record layout
  *                    [weight, PL_1..PL_L, steps] (io.cpp:302-310) with coalesced stores.
+ *   mcml_refshape_*  the same photon step in the reference's own launch structure (one
+ *                    photon per thread, 50 000-step relaunches), kept as a measured
+ *                    comparison for the persistent kernel.
  *
  * Arithmetic: only correctly rounded IEEE binary32 operations, the deterministic
  * functions of mcml_detmath.h and explicit operation order, compiled with
@@ -618,6 +621,107 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_pl(FirstPassArgs a)
     first_pass_body<true>(a, lds_raw);
 }
 
+/* ------------------------------------------------------- reference launch shape */
+/* The reference's own launch structure (transport.cu:100-147, 341-471) around this
+ * library's photon step, kept to measure what that structure costs on this GPU beside the
+ * persistent wavefront kernel: one photon per thread, no refill compaction and no boundary
+ * batching (a boundary hit and an interaction diverge within the step), each termination
+ * claiming the next photon with its own atomic, at most `steps` steps per launch with the
+ * photon kept in global memory between launches, and detections written to a per-thread
+ * buffer that the host copies back and reduces after every launch.  Photon k draws from
+ * stream (seed, 0, k) as in the persistent kernel, so the detections are the same. */
+namespace {
+struct RefSlot {
+    Photon p;
+    uint64_t k; /* photon index: the reference keeps the photon's initial curand state */
+};
+} // namespace
+
+/* LaunchPhoton_Global (transport.cu:722-739): photon k starts in slot k. */
+extern "C" __global__ void __launch_bounds__(256) mcml_refshape_init(RefShapeArgs a)
+{
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= a.threads) return;
+    const bool on = tid < a.total;
+    if (on) {
+        RefSlot* slot = static_cast<RefSlot*>(a.state) + tid;
+        Photon p;
+        launch(p, a.set, a.seed, 0u, tid, a.mult);
+        slot->p = p;
+        slot->k = tid;
+    }
+    a.active[tid] = on ? 1u : 0u;
+}
+
+/* MCd (transport.cu:341-471) with this library's hop / interaction / boundary steps. */
+extern "C" __global__ void __launch_bounds__(256) mcml_refshape_pass(RefShapeArgs a)
+{
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= a.threads) return;
+    RefFibers* f = a.fibers + tid;
+    int nf = 0;
+    uint32_t c_photons = 0, c_steps = 0, c_bnd = 0, n_int = 0;
+    if (a.active[tid]) {
+        RefSlot* slot = static_cast<RefSlot*>(a.state) + tid;
+        Photon p = slot->p;
+        uint64_t k = slot->k;
+        const DevSet* S = a.set;
+        GridLane gl = {nullptr, nullptr, nullptr, ~0ull, 0, 0, ~0u};
+        bool live = true;
+        for (int ii = 0; ii < a.steps; ii++) {
+            float s;
+            int nl;
+            uint32_t mask = 0;
+            int st;
+            if (hop<false>(p, s, nl, nullptr, 0)) {
+                c_bnd++;
+                st = cross<false>(p, nl, S, 0, mask, nullptr, 0);
+            } else {
+                st = interact<false>(p, s, gl, nullptr, n_int);
+            }
+            if (st == ST_ALIVE) continue;
+            c_steps += (uint32_t)p.steps;
+            if (st == ST_DETECTED) { /* detect(), transport.cu:872-938 */
+                for (int i = 1; i <= S->num_sds; i++) {
+                    if (!(mask & (1u << i))) continue;
+                    const float w = credited(p, S, i);
+                    const uint64_t key = k | ((uint64_t)i << MCML_KEY_SDS_SHIFT);
+                    if (nf < MCML_REF_SLOTS) {
+                        f->sds[nf] = i;
+                        f->w[nf] = w;
+                        f->key[nf] = key;
+                        nf++;
+                    } else {
+                        const unsigned long long idx = atomicAdd(a.n_spill, 1ull);
+                        if (idx < a.spill_cap) {
+                            a.spill_key[idx] = key;
+                            a.spill_w[idx] = w;
+                        }
+                    }
+                }
+            }
+            /* transport.cu:439-452: relaunch while photons remain, else retire the slot */
+            const unsigned long long t = atomicAdd(a.terminated, 1ull);
+            if (a.threads + t < a.total) {
+                k = a.threads + t;
+                launch(p, S, a.seed, 0u, k, a.mult);
+                c_photons++;
+            } else {
+                live = false;
+                break;
+            }
+        }
+        slot->p = p;
+        slot->k = k;
+        if (!live) a.active[tid] = 0u;
+    }
+    f->count = nf;
+    if (c_photons) atomicAdd(&a.counters[CNT_PHOTONS], (unsigned long long)c_photons);
+    if (c_steps) atomicAdd(&a.counters[CNT_STEPS], (unsigned long long)c_steps);
+    if (c_bnd) atomicAdd(&a.counters[CNT_BOUNDARY], (unsigned long long)c_bnd);
+    if (n_int) atomicAdd(&a.counters[CNT_INTERACTION], (unsigned long long)n_int);
+}
+
 /* ----------------------------------------------------------------- replay */
 #define REPLAY_BLOCK 256
 
@@ -938,3 +1042,19 @@ hipError_t mcml_occupancy_replay(int* blocks_per_cu)
 }
 
 int mcml_replay_block() { return REPLAY_BLOCK; }
+
+size_t mcml_refshape_slot_bytes() { return sizeof(RefSlot); }
+
+hipError_t mcml_launch_refshape_init(const RefShapeArgs& a, int block, hipStream_t st)
+{
+    const uint64_t grid = (a.threads + (uint64_t)block - 1) / (uint64_t)block;
+    hipLaunchKernelGGL(mcml_refshape_init, dim3((unsigned)grid), dim3(block), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t mcml_launch_refshape_pass(const RefShapeArgs& a, int block, hipStream_t st)
+{
+    const uint64_t grid = (a.threads + (uint64_t)block - 1) / (uint64_t)block;
+    hipLaunchKernelGGL(mcml_refshape_pass, dim3((unsigned)grid), dim3(block), 0, st, a);
+    return hipGetLastError();
+}
