@@ -10,15 +10,17 @@
  * Contents
  *   - multiply-with-carry uniform generator (Marsaglia 2003) with per-photon seeding by the
  *     SplitMix64 finaliser (Steele, Lea & Flood, OOPSLA 2014);
- *   - logarithm: exponent split, a fitted polynomial for log(1+f) on
- *     [sqrt(1/2)-1, sqrt(2)-1] (fit_log1p.py), Cody-Waite split of ln 2;
- *   - sin/cos of 2*pi*u: exact quarter-turn reduction + near-minimax polynomials on
- *     |a| <= pi/4 (fit_sincos.py; reduction after Cody & Waite, "Software Manual for the
- *     Elementary Functions", 1980);
+ *   - the step length -ln(u) and the azimuth's sin/cos straight from a 32-bit draw, both
+ *     table-driven (tables in mcml_tables.h, generated by gen_tables.py; the callers pass
+ *     them, the GPU kernels from LDS): Tang's logarithm (ACM TOMS 16(4), 1990) with a
+ *     Cody-Waite split of ln 2, and angle addition from 256 tabulated angles;
+ *   - sin/cos of 2*pi*u for launch angles: exact quarter-turn reduction + near-minimax
+ *     polynomials on |a| <= pi/4 (fit_sincos.py; reduction after Cody & Waite, "Software
+ *     Manual for the Elementary Functions", 1980);
  *   - acos: asin Taylor series (A&S 4.4.40) on [0, 1/2] and acos(x) = 2 asin(sqrt((1-x)/2));
  *   - 1/x and 1/sqrt(x): bit-pattern seeds and three Newton steps.
  * The asin coefficients are the exact rational Taylor coefficients rounded to binary32; the
- * log and sin/cos coefficients come from fit_log1p.py and fit_sincos.py.
+ * launch sin/cos coefficients come from fit_sincos.py.
  * None of these reproduce any vendor math library; they are accurate to a few ulp and,
  * more importantly, give identical bits on the host and on the GPU.
  */
@@ -122,72 +124,39 @@ MCML_HD mcml_mwc mcml_stream(uint64_t seed, uint32_t set, uint64_t k, const uint
     return r;
 }
 
-/* ---------------------------------------------------------------- logarithm */
+/* ---------------------------------------------------------------- step length */
 #define MCML_LN2_HI 0.693359375f       /* 355/512, exact product with small integers */
 #define MCML_LN2_LO -2.12194440e-4f    /* ln 2 - LN2_HI */
 
-/* Natural logarithm of a positive finite float: x = 2^e * m with m in (sqrt(1/2), sqrt(2)],
- * log(m) = f + f^2 P(f), f = m - 1, P of degree 8 fitted for minimal maximum relative error
- * on that interval (fit_log1p.py; 4e-9 before rounding the coefficients to binary32). */
-MCML_HD float mcml_log(float x)
+/* -ln(u) for the uniform u = n 2^-24, n = (x >> 8) + 1 in [1, 2^24], from one 32-bit draw x:
+ * the exponential step of the photon walk.  Table-driven (Tang, ACM TOMS 16(4), 1990):
+ * float(n) = 2^e m is exact, m in [1, 2) lies in cell j of 128 (its top seven fraction
+ * bits); with the tabulated c_j ~ 1/m and L_j = -ln(c_j) (mcml_tables.h, `tab` =
+ * mcml_lg_table), r = m c_j - 1 (one fma, |r| < 1/256) and
+ * ln u = (e - 24) ln 2 + L_j + ln(1 + r),  ln(1 + r) = r + r^2 (r/3 - 1/2)  (error < r^4/4).
+ * Over all 2^24 inputs the error is below 7.2e-8 absolute where -ln u < 1 and 8.9e-8
+ * relative above (tests/test_detmath.py checks every n).  For u within a few ulp of 1 the
+ * sum cancels, so the result is clamped at 0: a step is never negative. */
+MCML_HD float mcml_neglog_u32(uint32_t x, const float* tab)
 {
-    uint32_t ix = mcml_f2u(x);
-    int32_t e = 0;
-    if (ix < 0x00800000u) { /* subnormal: scale by 2^25 first */
-        x = x * 33554432.0f;
-        ix = mcml_f2u(x);
-        e = -25;
-    }
-    e += (int32_t)(ix >> 23) - 127;
-    uint32_t mb = (ix & 0x007FFFFFu) | 0x3F800000u;
-    if (mb > 0x3FB504F3u) { /* m > sqrt(2): halve it so m lies in (sqrt(1/2), sqrt(2)] */
-        mb -= 0x00800000u;
-        e += 1;
-    }
-    float f = mcml_u2f(mb) - 1.0f; /* exact */
-    float p = fmaf(f, -7.623060793e-02f, 1.290550232e-01f);
-    p = fmaf(f, p, -1.324298084e-01f);
-    p = fmaf(f, p, 1.418163478e-01f);
-    p = fmaf(f, p, -1.660905629e-01f);
-    p = fmaf(f, p, 2.000198513e-01f);
-    p = fmaf(f, p, -2.500158548e-01f);
-    p = fmaf(f, p, 3.333332539e-01f);
-    p = fmaf(f, p, -4.999998808e-01f);
-    float lm = fmaf(f * f, p, f);
-    float fe = (float)e;
-    return fmaf(fe, MCML_LN2_HI, fmaf(fe, MCML_LN2_LO, lm));
-}
-
-/* log(u) for the generator's outputs u in [2^-23, 1] (never subnormal): the same
- * arithmetic as mcml_log without the subnormal rescaling. */
-MCML_HD float mcml_log_unit(float x)
-{
-    uint32_t ix = mcml_f2u(x);
-    int32_t e = (int32_t)(ix >> 23) - 127;
-    uint32_t mb = (ix & 0x007FFFFFu) | 0x3F800000u;
-    if (mb > 0x3FB504F3u) {
-        mb -= 0x00800000u;
-        e += 1;
-    }
-    float f = mcml_u2f(mb) - 1.0f;
-    float p = fmaf(f, -7.623060793e-02f, 1.290550232e-01f);
-    p = fmaf(f, p, -1.324298084e-01f);
-    p = fmaf(f, p, 1.418163478e-01f);
-    p = fmaf(f, p, -1.660905629e-01f);
-    p = fmaf(f, p, 2.000198513e-01f);
-    p = fmaf(f, p, -2.500158548e-01f);
-    p = fmaf(f, p, 3.333332539e-01f);
-    p = fmaf(f, p, -4.999998808e-01f);
-    float lm = fmaf(f * f, p, f);
-    float fe = (float)e;
-    return fmaf(fe, MCML_LN2_HI, fmaf(fe, MCML_LN2_LO, lm));
+    const uint32_t n = (x >> 8) + 1u;
+    const uint32_t b = mcml_f2u((float)n);
+    const uint32_t j = (b >> 16) & 127u;
+    const float m = mcml_u2f((b & 0x007FFFFFu) | 0x3F800000u);
+    const float le = (float)((int32_t)(b >> 23) - 151);
+    const float c = tab[2 * j], L = tab[2 * j + 1];
+    const float r = fmaf(m, c, -1.0f);
+    const float q = fmaf(r, 0.333333343f, -0.5f);
+    const float p = fmaf(q * r, r, r);
+    const float lg = fmaf(le, MCML_LN2_HI, fmaf(le, MCML_LN2_LO, L + p));
+    return fmaxf(-lg, 0.0f);
 }
 
 /* ---------------------------------------------------------------- sin / cos */
 #define MCML_TWO_PI 6.28318530717958647692f
 #define MCML_INV_TWO_PI 0.159154943091895335768f
 
-/* sin(2*pi*t) and cos(2*pi*t) for 0 <= t <= 1 (any |t| < 2^22 works). */
+/* sin(2*pi*t) and cos(2*pi*t) for 0 <= t <= 1 (any |t| < 2^22 works); used at launch. */
 MCML_HD void mcml_sincos_turns(float t, float* sn, float* cs)
 {
     float q = rintf(t * 4.0f);          /* nearest quarter turn; t*4 is exact */
@@ -213,6 +182,23 @@ MCML_HD void mcml_sincos_turns(float t, float* sn, float* cs)
     rc = mcml_u2f(mcml_f2u(rc) ^ ((uint32_t)((k + 1) & 2) << 30));
     *sn = rs;
     *cs = rc;
+}
+
+/* sin and cos of the azimuth 2 pi x / 2^32 for one 32-bit draw x.  Table-driven: the top
+ * eight bits j pick (cos, sin) of 2 pi j / 256 (mcml_tables.h, `tab` = mcml_sc_table), the
+ * low 24 bits give the remaining angle d < 2 pi / 256 = 0.0246, and angle addition with
+ * sin d = d - d^3/6 (error < d^5/120 = 7.4e-11) and cos d = 1 - d^2/2 (error < d^4/24 =
+ * 1.5e-8) gives both results within 1.2e-7 (tests/test_detmath.py). */
+MCML_HD void mcml_sincos_u32(uint32_t x, const float* tab, float* sn, float* cs)
+{
+    const uint32_t j = x >> 24;
+    const float d = (float)(x & 0x00FFFFFFu) * 1.46291807926715968e-9f; /* 2 pi / 2^32 */
+    const float d2 = d * d;
+    const float sd = fmaf(d * -0.166666672f, d2, d);
+    const float cd = fmaf(d2, -0.5f, 1.0f);
+    const float ca = tab[2 * j], sa = tab[2 * j + 1];
+    *sn = fmaf(sa, cd, ca * sd);
+    *cs = fmaf(ca, cd, -(sa * sd));
 }
 
 /* sin and cos of an angle in radians, |a| <= 4 (used for launch angles only). */

@@ -28,10 +28,24 @@
 #include "mcml_detmath.h"
 #include "mcml_device.h"
 #include "mwc_multipliers.h"
+#include "mcml_tables.h"
 
 #define RPI_F 0.318309886f /* 1/pi as the reference writes it (header.h:41) */
 
 namespace {
+
+/* The step's two lookup tables (azimuth sin/cos, logarithm cells), copied into LDS by every
+ * workgroup of a kernel that moves photons before its first step. */
+__shared__ float g_tab[2 * MCML_SC_N + 2 * MCML_LG_N];
+#define SC_TAB (g_tab)
+#define LG_TAB (g_tab + 2 * MCML_SC_N)
+
+__device__ __forceinline__ void stage_tables()
+{
+    for (int i = threadIdx.x; i < 2 * MCML_SC_N; i += blockDim.x) g_tab[i] = mcml_sc_table[i];
+    for (int i = threadIdx.x; i < 2 * MCML_LG_N; i += blockDim.x) g_tab[2 * MCML_SC_N + i] = mcml_lg_table[i];
+    __syncthreads();
+}
 
 /* ----------------------------------------------------------------- wave helpers */
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -142,7 +156,7 @@ __device__ __forceinline__ void spin(Photon& p)
     const bool iso = p.hg_c == 0.0f; /* g == 0 */
     const float D = iso ? 1.0f : fmaf(p.hg_c, u, p.hg_b); /* HG denominator 1 - g + 2gu */
     float sinp, cosp;
-    mcml_sincos_turns(mcml_uniform(&p.rng), &sinp, &cosp);
+    mcml_sincos_u32(mcml_mwc_next(&p.rng), SC_TAB, &sinp, &cosp);
     const float b = fmaf(-p.dz, p.dz, 1.0f);
     if (b == 0.0f) {
         float cost = 2.0f * u - 1.0f;
@@ -273,7 +287,7 @@ enum StepOutcome { ST_ALIVE = 0, ST_DEAD = 1, ST_DETECTED = 2 };
 template <bool REPLAY>
 __device__ __forceinline__ bool hop(Photon& p, float& s, int& nl, float* pl_lane, int pl_stride)
 {
-    s = -mcml_log_unit(mcml_uniform(&p.rng)) * p.mutr;
+    s = mcml_neglog_u32(mcml_mwc_next(&p.rng), LG_TAB) * p.mutr;
     nl = p.layer;
     if (fmaf(s, p.dz, p.z) < p.zmin) {
         nl--;
@@ -609,6 +623,7 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
 
 extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass(FirstPassArgs a)
 {
+    stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
     first_pass_body<false>(a, lds_raw);
 }
@@ -617,6 +632,7 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass(FirstPassArgs a)
  * second pass: a detected photon's first-pass trajectory is its replay trajectory). */
 extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_pl(FirstPassArgs a)
 {
+    stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
     first_pass_body<true>(a, lds_raw);
 }
@@ -656,6 +672,7 @@ extern "C" __global__ void __launch_bounds__(256) mcml_refshape_init(RefShapeArg
 /* MCd (transport.cu:341-471) with this library's hop / interaction / boundary steps. */
 extern "C" __global__ void __launch_bounds__(256) mcml_refshape_pass(RefShapeArgs a)
 {
+    stage_tables();
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= a.threads) return;
     RefFibers* f = a.fibers + tid;
@@ -745,6 +762,7 @@ __device__ __forceinline__ void flush_tile(unsigned long long* tile, uint32_t gr
 
 extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArgs a)
 {
+    stage_tables();
     /* per-lane pathlength accumulators, layer-major so lanes never share a bank word */
     __shared__ float pl_lds[MCML_MAX_LAYERS * REPLAY_BLOCK];
     __shared__ unsigned long long grid_tiles[REPLAY_BLOCK / 64][TILE_WORDS];

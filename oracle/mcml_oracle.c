@@ -33,6 +33,7 @@
 #include "../include/mcml.h"
 #include "../mcml_gpu_amd/csrc/mcml_detmath.h"
 #include "../mcml_gpu_amd/csrc/mwc_multipliers.h"
+#include "../mcml_gpu_amd/csrc/mcml_tables.h"
 
 #define O_PI 3.141592654f  /* header.h:40 */
 #define O_RPI 0.318309886f /* header.h:41 */
@@ -166,7 +167,7 @@ float oracle_spin(oracle_photon* p, float g)
     const int iso = g == 0.0f;
     const float D = iso ? 1.0f : fmaf(2.0f * g, u, 1.0f - g); /* HG denominator 1 - g + 2gu */
     float sinp, cosp;
-    mcml_sincos_turns(urand(p), &sinp, &cosp);
+    mcml_sincos_u32(mcml_mwc_next(&p->rng), mcml_sc_table, &sinp, &cosp);
     float b = fmaf(-p->dz, p->dz, 1.0f);
     float cost;
     if (b == 0.0f) { /* travelling along z */
@@ -330,7 +331,7 @@ uint32_t oracle_photon_life(const mcml_derived* d, oracle_photon* p, int target,
     uint32_t detected = 0;
     for (;;) {
         const mcml_layer_derived* Ly = &d->layer[p->layer];
-        float s = -mcml_log_unit(urand(p)) * Ly->mutr; /* transport.cu:373 */
+        float s = mcml_neglog_u32(mcml_mwc_next(&p->rng), mcml_lg_table) * Ly->mutr; /* transport.cu:373 */
         int new_layer = p->layer;
         if (fmaf(s, p->dz, p->z) < Ly->z_min) { /* transport.cu:379-386 */
             new_layer--;
@@ -510,13 +511,20 @@ const uint32_t* oracle_multiplier_table(uint32_t* n)
     *n = MCML_MWC_TABLE_SIZE;
     return mcml_mwc_multipliers;
 }
-void oracle_log_n(const float* x, float* y, int64_t n)
+void oracle_neglog_u32_n(const uint32_t* x, float* y, int64_t n)
 {
-    for (int64_t i = 0; i < n; i++) y[i] = mcml_log(x[i]);
+    for (int64_t i = 0; i < n; i++) y[i] = mcml_neglog_u32(x[i], mcml_lg_table);
 }
-void oracle_log_unit_n(const float* x, float* y, int64_t n)
+void oracle_sincos_u32_n(const uint32_t* x, float* s, float* c, int64_t n)
 {
-    for (int64_t i = 0; i < n; i++) y[i] = mcml_log_unit(x[i]);
+    for (int64_t i = 0; i < n; i++) mcml_sincos_u32(x[i], mcml_sc_table, &s[i], &c[i]);
+}
+void oracle_tables(const float** sc, int* n_sc, const float** lg, int* n_lg)
+{
+    *sc = mcml_sc_table;
+    *n_sc = MCML_SC_N;
+    *lg = mcml_lg_table;
+    *n_lg = MCML_LG_N;
 }
 void oracle_sincos_turns_n(const float* t, float* s, float* c, int64_t n)
 {
@@ -597,7 +605,7 @@ void oracle_steps_n(const mcml_derived* d, int layer, uint64_t seed, float* out,
     oracle_photon p;
     memset(&p, 0, sizeof(p));
     p.rng = mcml_stream(seed, 0, 3, mcml_mwc_multipliers, MCML_MWC_TABLE_SIZE);
-    for (int64_t i = 0; i < n; i++) out[i] = -mcml_log_unit(urand(&p)) * d->layer[layer].mutr;
+    for (int64_t i = 0; i < n; i++) out[i] = mcml_neglog_u32(mcml_mwc_next(&p.rng), mcml_lg_table) * d->layer[layer].mutr;
 }
 /* Detection test on an explicit exit state (x, y, dx, dy, dz, w); returns the mask. */
 uint32_t oracle_detect_state(const mcml_derived* d, const float* st, int only, float* wout)

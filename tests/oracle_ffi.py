@@ -43,8 +43,11 @@ def lib():
                                       C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
     L.oracle_multiplier_table.argtypes = [C.POINTER(C.c_uint32)]
     L.oracle_multiplier_table.restype = C.POINTER(C.c_uint32)
-    L.oracle_log_n.argtypes = [_fp, _fp, C.c_int64]
-    L.oracle_log_unit_n.argtypes = [_fp, _fp, C.c_int64]
+    _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+    L.oracle_neglog_u32_n.argtypes = [_u32p, _fp, C.c_int64]
+    L.oracle_sincos_u32_n.argtypes = [_u32p, _fp, _fp, C.c_int64]
+    L.oracle_tables.argtypes = [C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_int),
+                                C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_int)]
     L.oracle_sincos_turns_n.argtypes = [_fp, _fp, _fp, C.c_int64]
     L.oracle_acos_n.argtypes = [_fp, _fp, C.c_int64]
     L.oracle_rcp_n.argtypes = [_fp, _fp, C.c_int64]

@@ -13,33 +13,70 @@ def ulp_err(y32, ref64):
     return np.abs(y - ref64) / sp
 
 
-def test_log_all_generator_outputs():
-    # every value the generator can produce: 1 - k * 2^-23, k = 2^23 - 1 .. 0 (ascending)
-    k = np.arange(2 ** 23 - 1, -1, -1, dtype=np.float64)
-    u = (1.0 - k * 2.0 ** -23).astype(np.float32)
-    assert np.array_equal(u.astype(np.float64), 1.0 - k * 2.0 ** -23)  # exact in binary32
-    y = np.zeros_like(u)
-    O.lib().oracle_log_unit_n(u, y, len(u))
-    ref = np.log(u.astype(np.float64))
-    nz = ref != 0
-    assert ulp_err(y[nz], ref[nz]).max() < 1.5
-    assert y[-1] == 0.0  # log(1) = 0 exactly: u = 1 gives a zero-length step
-    y2 = np.zeros_like(u)
-    O.lib().oracle_log_n(u, y2, len(u))
-    assert np.array_equal(y.view(np.uint32), y2.view(np.uint32))
+def _tables():
+    import ctypes as C
+    sc, lg = C.POINTER(C.c_float)(), C.POINTER(C.c_float)()
+    nsc, nlg = C.c_int(0), C.c_int(0)
+    O.lib().oracle_tables(C.byref(sc), C.byref(nsc), C.byref(lg), C.byref(nlg))
+    return (np.ctypeslib.as_array(sc, (2 * nsc.value,)).copy(),
+            np.ctypeslib.as_array(lg, (2 * nlg.value,)).copy())
 
 
-def test_log_general_and_subnormal():
-    rng = np.random.default_rng(3)
-    x = np.concatenate([
-        rng.uniform(1e-3, 1e3, 200000), 10.0 ** rng.uniform(-37, 38, 200000),
-        np.array([1.0, 2.0, 0.5, np.sqrt(2.0), 1.4142135, 1.4142137]),
-        np.array([1e-40, 1e-45, 3e-39, 1.1754942e-38])]).astype(np.float32)
-    y = np.zeros_like(x)
-    O.lib().oracle_log_n(x, y, len(x))
-    ref = np.log(x.astype(np.float64))
-    nz = ref != 0
-    assert ulp_err(y[nz], ref[nz]).max() < 1.5
+def test_tables_are_their_formulas():
+    """Every table entry is the float64 formula rounded to nearest binary32: cos/sin of
+    2 pi j / 256, and c_j = 1 / (1 + (j + 1/2) / 128) with L_j = -ln(c_j)."""
+    sc, lg = _tables()
+    j = np.arange(256)
+    a = 2 * np.pi * j / 256
+    assert np.array_equal(sc[0::2], np.cos(a).astype(np.float32))
+    assert np.array_equal(sc[1::2], np.sin(a).astype(np.float32))
+    j = np.arange(128)
+    c = (1.0 / (1.0 + (j + 0.5) / 128.0)).astype(np.float32)
+    assert np.array_equal(lg[0::2], c)
+    assert np.array_equal(lg[1::2], (-np.log(c.astype(np.float64))).astype(np.float32))
+
+
+def neglog(x):
+    y = np.zeros(len(x), np.float32)
+    O.lib().oracle_neglog_u32_n(np.ascontiguousarray(x, np.uint32), y, len(x))
+    return y
+
+
+def test_neglog_every_step_value():
+    """-ln(u) for all 2^24 values u = n 2^-24 the step can draw (x = (n - 1) << 8): within
+    7.5e-8 + 9e-8 |ln u| of float64 (7.2e-8 absolute below 1, 8.9e-8 relative above),
+    never negative, exactly 0 at u = 1, and the largest step is -ln(2^-24) = 16.64 mean
+    free paths."""
+    n = np.arange(1, 2 ** 24 + 1, dtype=np.uint64)
+    x = ((n - 1) << 8).astype(np.uint32)
+    y = neglog(x)
+    ref = -np.log(n.astype(np.float64) * 2.0 ** -24)
+    err = np.abs(y.astype(np.float64) - ref)
+    assert (err <= 7.5e-8 + 9e-8 * ref).all(), err.max()
+    assert (y >= 0).all() and y[-1] == 0.0
+    assert abs(y[0] - 24 * np.log(2)) < 2e-6
+    # the low 8 bits of the draw are not used
+    lo = neglog((x[::4097] | np.uint32(0xA5)))
+    assert np.array_equal(lo.view(np.uint32), y[::4097].view(np.uint32))
+
+
+def test_sincos_u32():
+    """sin/cos of 2 pi x / 2^32 against float64, on random draws and at every cell edge."""
+    rng = np.random.default_rng(6)
+    j = np.arange(256, dtype=np.uint64)
+    x = np.concatenate([rng.integers(0, 2 ** 32, 2000000, dtype=np.uint64), j << 24, (j << 24) | 0xFFFFFF,
+                        (j << 24) | 0x800000]).astype(np.uint32)
+    s = np.zeros(len(x), np.float32)
+    c = np.zeros(len(x), np.float32)
+    O.lib().oracle_sincos_u32_n(x, s, c, len(x))
+    a = 2 * np.pi * x.astype(np.float64) / 2 ** 32
+    assert np.abs(s - np.sin(a)).max() < 1.2e-7
+    assert np.abs(c - np.cos(a)).max() < 1.2e-7
+    n2 = s.astype(np.float64) ** 2 + c.astype(np.float64) ** 2
+    assert np.abs(n2 - 1).max() < 3e-7
+    # cell starts are the table entries themselves
+    k = len(x) - 3 * 256
+    assert np.array_equal(s[k:k + 256], np.sin(2 * np.pi * np.arange(256) / 256).astype(np.float32))
 
 
 def test_sincos_turns():
