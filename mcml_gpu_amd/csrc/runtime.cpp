@@ -33,7 +33,7 @@ namespace {
 constexpr float kPI = 3.141592654f; /* header.h:40 */
 /* launch defaults (results never depend on them: each photon's stream is fixed) */
 constexpr int kDefaultChunk = 128;
-constexpr int kDefaultRefill = 2;
+constexpr int kDefaultRefill = 4;
 constexpr int kDefaultBoundary = 4;
 constexpr size_t kLdsTableMax = 16384;
 constexpr int kCtrlWords = 8;
@@ -948,7 +948,7 @@ extern "C" int mcml_autotune(mcml_ctx* c, const mcml_sim* sims, int nsets, uint6
     const int blocks[] = {128, 256};
     /* (refill threshold, boundary batch) pairs: idle lanes to wait for before a refill, and
      * lanes at a boundary to wait for before resolving boundaries together */
-    const int rb[][2] = {{1, 1}, {2, 1}, {2, 4}, {4, 4}};
+    const int rb[][2] = {{2, 1}, {4, 1}, {2, 4}, {4, 4}, {4, 8}};
     const int chunks[] = {64, 128, 256};
     double best = 1e300;
     mcml_opts pick = *opts;
