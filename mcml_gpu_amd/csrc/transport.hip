@@ -154,7 +154,8 @@ __device__ __forceinline__ void spin(Photon& p)
 {
     const float u = mcml_uniform(&p.rng);
     const bool iso = p.hg_c == 0.0f; /* g == 0 */
-    const float D = iso ? 1.0f : fmaf(p.hg_c, u, p.hg_b); /* HG denominator 1 - g + 2gu */
+    /* HG denominator 1 - g + 2gu; exactly 1 for g = 0 (hg_c = 0, hg_b = 1) */
+    const float D = fmaf(p.hg_c, u, p.hg_b);
     float sinp, cosp;
     mcml_sincos_u32(mcml_mwc_next(&p.rng), SC_TAB, &sinp, &cosp);
     const float b = fmaf(-p.dz, p.dz, 1.0f);
@@ -289,21 +290,28 @@ __device__ __forceinline__ bool hop(Photon& p, float& s, int& nl, float* pl_lane
 {
     s = mcml_neglog_u32(mcml_mwc_next(&p.rng), LG_TAB) * p.mutr;
     nl = p.layer;
+    bool cut = false;
     if (fmaf(s, p.dz, p.z) < p.zmin) {
         nl--;
         s = (p.zmin - p.z) / p.dz;
+        cut = true;
     }
     if (fmaf(s, p.dz, p.z) > p.zmax) {
         nl++;
         s = (p.zmax - p.z) / p.dz;
+        cut = true;
     }
     p.x = fmaf(p.dx, s, p.x);
     p.y = fmaf(p.dy, s, p.y);
     p.z = fmaf(p.dz, s, p.z);
     p.steps++;
     if (REPLAY) p.pl_cur += s; /* transport.cu:529: pathlength of the layer the hop started in */
-    if (p.z > p.zmax) p.z = p.zmax;
-    if (p.z < p.zmin) p.z = p.zmin;
+    /* transport.cu:394-395.  An uncut step lands on the value both tests saw, which lies in
+     * [zmin, zmax], so only a step cut at a boundary can need the clamp. */
+    if (cut) {
+        if (p.z > p.zmax) p.z = p.zmax;
+        if (p.z < p.zmin) p.z = p.zmin;
+    }
     return nl != p.layer;
 }
 
