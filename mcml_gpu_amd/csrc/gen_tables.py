@@ -1,11 +1,11 @@
 #!/usr/bin/env python3
 """Generate the two lookup tables of the photon step (mcml_tables.h).
 
-mcml_sc_table[2*j], [2*j+1] = cos(2 pi j / 256), sin(2 pi j / 256), j = 0..255:
+mcml_sc_table[j] = {cos(2 pi j / 256), sin(2 pi j / 256)}, j = 0..255:
     the azimuth table of mcml_sincos_u32 (angle addition with a short polynomial for the
     remaining angle below 2 pi / 256).
 
-mcml_lg_table[2*j], [2*j+1] = c_j, L_j, j = 0..127, with
+mcml_lg_table[j] = {c_j, L_j}, j = 0..127, with
     c_j = binary32(1 / (1 + (j + 1/2) / 128))   (reciprocal of the centre of mantissa cell j)
     L_j = binary32(-ln(c_j))                    (logarithm of the rounded reciprocal)
     the table of mcml_neglog_u32, after Tang, "Table-driven implementation of the
@@ -52,6 +52,7 @@ def main():
     print(" * reciprocal / logarithm pairs of the %d mantissa cells of [1, 2). */" % LG_N)
     print("#ifndef MCML_TABLES_H")
     print("#define MCML_TABLES_H")
+    print('#include "mcml_detmath.h" /* mcml_pair */')
     print("#if defined(__HIPCC__)")
     print("#define MCML_TABLE __device__ static const")
     print("#else")
@@ -59,13 +60,13 @@ def main():
     print("#endif")
     print("#define MCML_SC_N %d" % SC_N)
     print("#define MCML_LG_N %d" % LG_N)
-    print("MCML_TABLE float mcml_sc_table[2 * MCML_SC_N] = {")
+    print("MCML_TABLE mcml_pair mcml_sc_table[MCML_SC_N] = {")
     for c, s in sc_entries():
-        print("    %s, %s," % (lit(c), lit(s)))
+        print("    {%s, %s}," % (lit(c), lit(s)))
     print("};")
-    print("MCML_TABLE float mcml_lg_table[2 * MCML_LG_N] = {")
+    print("MCML_TABLE mcml_pair mcml_lg_table[MCML_LG_N] = {")
     for c, L in lg_entries():
-        print("    %s, %s," % (lit(c), lit(L)))
+        print("    {%s, %s}," % (lit(c), lit(L)))
     print("};")
     print("#endif")
 

@@ -59,10 +59,12 @@ MCML_HD uint32_t mcml_mwc_next(mcml_mwc* r)
  * float 1.k in [1, 2) built from the bits (three integer/float operations, no conversion).
  * Never 0, so -log(u) is finite (the same half-open interval as the reference's
  * generator); the smallest value is 2^-23. */
+/* 1 + k 2^-23 in [1, 2) from the top 23 bits k of a draw: the uniform below is 2 minus it. */
+MCML_HD float mcml_one_plus(uint32_t x) { return mcml_u2f(0x3F800000u | (x >> 9)); }
+
 MCML_HD float mcml_uniform(mcml_mwc* r)
 {
-    uint32_t x = mcml_mwc_next(r);
-    return 2.0f - mcml_u2f(0x3F800000u | (x >> 9));
+    return 2.0f - mcml_one_plus(mcml_mwc_next(r));
 }
 
 /* 1/x for positive normal x: a bit-pattern seed (relative error below 5.1 %) refined by
@@ -125,6 +127,11 @@ MCML_HD mcml_mwc mcml_stream(uint64_t seed, uint32_t set, uint64_t k, const uint
 }
 
 /* ---------------------------------------------------------------- step length */
+/* One table entry: two floats read together (one 64-bit load from LDS on the GPU). */
+typedef struct __attribute__((aligned(8))) {
+    float a, b;
+} mcml_pair;
+
 #define MCML_LN2_HI 0.693359375f       /* 355/512, exact product with small integers */
 #define MCML_LN2_LO -2.12194440e-4f    /* ln 2 - LN2_HI */
 
@@ -137,14 +144,15 @@ MCML_HD mcml_mwc mcml_stream(uint64_t seed, uint32_t set, uint64_t k, const uint
  * Over all 2^24 inputs the error is below 7.2e-8 absolute where -ln u < 1 and 8.9e-8
  * relative above (tests/test_detmath.py checks every n).  For u within a few ulp of 1 the
  * sum cancels, so the result is clamped at 0: a step is never negative. */
-MCML_HD float mcml_neglog_u32(uint32_t x, const float* tab)
+MCML_HD float mcml_neglog_u32(uint32_t x, const mcml_pair* tab)
 {
     const uint32_t n = (x >> 8) + 1u;
     const uint32_t b = mcml_f2u((float)n);
     const uint32_t j = (b >> 16) & 127u;
     const float m = mcml_u2f((b & 0x007FFFFFu) | 0x3F800000u);
     const float le = (float)((int32_t)(b >> 23) - 151);
-    const float c = tab[2 * j], L = tab[2 * j + 1];
+    const mcml_pair t = tab[j];
+    const float c = t.a, L = t.b;
     const float r = fmaf(m, c, -1.0f);
     const float q = fmaf(r, 0.333333343f, -0.5f);
     const float p = fmaf(q * r, r, r);
@@ -189,14 +197,15 @@ MCML_HD void mcml_sincos_turns(float t, float* sn, float* cs)
  * low 24 bits give the remaining angle d < 2 pi / 256 = 0.0246, and angle addition with
  * sin d = d - d^3/6 (error < d^5/120 = 7.4e-11) and cos d = 1 - d^2/2 (error < d^4/24 =
  * 1.5e-8) gives both results within 1.2e-7 (tests/test_detmath.py). */
-MCML_HD void mcml_sincos_u32(uint32_t x, const float* tab, float* sn, float* cs)
+MCML_HD void mcml_sincos_u32(uint32_t x, const mcml_pair* tab, float* sn, float* cs)
 {
     const uint32_t j = x >> 24;
     const float d = (float)(x & 0x00FFFFFFu) * 1.46291807926715968e-9f; /* 2 pi / 2^32 */
     const float d2 = d * d;
     const float sd = fmaf(d * -0.166666672f, d2, d);
     const float cd = fmaf(d2, -0.5f, 1.0f);
-    const float ca = tab[2 * j], sa = tab[2 * j + 1];
+    const mcml_pair t = tab[j];
+    const float ca = t.a, sa = t.b;
     *sn = fmaf(sa, cd, ca * sd);
     *cs = fmaf(ca, cd, -(sa * sd));
 }

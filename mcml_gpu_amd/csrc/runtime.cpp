@@ -149,8 +149,8 @@ void fill_devset(const mcml_derived& d, DevSet& S)
         o.albedo = in.mua * in.mutr;
         o.g = in.g;
         o.hg_a = 1.0f - in.g * in.g;
-        o.hg_b = 1.0f - in.g;
-        o.hg_c = 2.0f * in.g;
+        o.hg_e = 2.0f * (2.0f * in.g) + (1.0f - in.g);
+        o.hg_cn = -(2.0f * in.g);
         o.hg_d = 1.0f + in.g * in.g;
         o.hg_inv2g = 1.0f / (2.0f * in.g);
         o.n = in.n;

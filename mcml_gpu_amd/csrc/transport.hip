@@ -36,14 +36,14 @@ namespace {
 
 /* The step's two lookup tables (azimuth sin/cos, logarithm cells), copied into LDS by every
  * workgroup of a kernel that moves photons before its first step. */
-__shared__ float g_tab[2 * MCML_SC_N + 2 * MCML_LG_N];
+__shared__ mcml_pair g_tab[MCML_SC_N + MCML_LG_N];
 #define SC_TAB (g_tab)
-#define LG_TAB (g_tab + 2 * MCML_SC_N)
+#define LG_TAB (g_tab + MCML_SC_N)
 
 __device__ __forceinline__ void stage_tables()
 {
-    for (int i = threadIdx.x; i < 2 * MCML_SC_N; i += blockDim.x) g_tab[i] = mcml_sc_table[i];
-    for (int i = threadIdx.x; i < 2 * MCML_LG_N; i += blockDim.x) g_tab[2 * MCML_SC_N + i] = mcml_lg_table[i];
+    for (int i = threadIdx.x; i < MCML_SC_N; i += blockDim.x) g_tab[i] = mcml_sc_table[i];
+    for (int i = threadIdx.x; i < MCML_LG_N; i += blockDim.x) g_tab[MCML_SC_N + i] = mcml_lg_table[i];
     __syncthreads();
 }
 
@@ -80,7 +80,7 @@ struct Photon {
     int32_t first;
     mcml_mwc rng;
     /* the current layer's parameters, reloaded only when the layer changes */
-    float zmin, zmax, mutr, albedo, hg_a, hg_b, hg_c, hg_d, hg_inv2g; /* hg_c = 2g: 0 = isotropic */
+    float zmin, zmax, mutr, albedo, hg_a, hg_e, hg_cn, hg_d, hg_inv2g; /* hg_cn = -2g: 0 = isotropic */
     /* running pathlength in the current layer; the per-layer totals live in LDS and are
      * exchanged with this register only when the photon changes layer */
     float pl_cur;
@@ -96,8 +96,8 @@ __device__ __forceinline__ void load_layer(Photon& p, const DevSet* S)
     p.mutr = a.z;
     p.albedo = a.w;
     p.hg_a = b.y;
-    p.hg_b = b.z;
-    p.hg_c = b.w;
+    p.hg_e = b.z;
+    p.hg_cn = b.w;
     p.hg_d = c.x;
     p.hg_inv2g = c.y;
 }
@@ -152,15 +152,16 @@ __device__ __forceinline__ void launch(Photon& p, const DevSet* S, uint64_t seed
  * rsqrt about 1 for the renormalisation (the norm never drifts more than a few ulp). */
 __device__ __forceinline__ void spin(Photon& p)
 {
-    const float u = mcml_uniform(&p.rng);
-    const bool iso = p.hg_c == 0.0f; /* g == 0 */
-    /* HG denominator 1 - g + 2gu; exactly 1 for g = 0 (hg_c = 0, hg_b = 1) */
-    const float D = fmaf(p.hg_c, u, p.hg_b);
+    /* the HG uniform u = 2 - v, v = 1.k in [1, 2): D = 1 - g + 2gu = (1 + 3g) - 2gv, and the
+     * isotropic cos(theta) = 2u - 1 = 3 - 2v exactly */
+    const float v = mcml_one_plus(mcml_mwc_next(&p.rng));
+    const bool iso = p.hg_cn == 0.0f; /* g == 0 */
+    const float D = fmaf(p.hg_cn, v, p.hg_e); /* exactly 1 for g = 0 */
     float sinp, cosp;
     mcml_sincos_u32(mcml_mwc_next(&p.rng), SC_TAB, &sinp, &cosp);
     const float b = fmaf(-p.dz, p.dz, 1.0f);
     if (b == 0.0f) {
-        float cost = 2.0f * u - 1.0f;
+        float cost = fmaf(-2.0f, v, 3.0f);
         if (!iso) {
             const float temp = p.hg_a / D;
             cost = fmaf(-temp, temp, p.hg_d) * p.hg_inv2g;
@@ -172,7 +173,7 @@ __device__ __forceinline__ void spin(Photon& p)
     } else {
         /* one division serves the HG inversion (1/D = b r) and the rotation (1/b = D r) */
         const float r = mcml_rcp(D * b);
-        float cost = 2.0f * u - 1.0f;
+        float cost = fmaf(-2.0f, v, 3.0f);
         if (!iso) {
             const float temp = p.hg_a * (b * r);
             cost = fmaf(-temp, temp, p.hg_d) * p.hg_inv2g;

@@ -156,22 +156,25 @@ void oracle_launch(const mcml_derived* d, uint64_t seed, uint32_t set, uint64_t 
  *   cos(theta) = (1 + g^2 - t^2) * (1 / 2g)              (reference divides by 2g)
  *   P = sqrt((1 - cos^2)(1 - dz^2)), Q = P / (1 - dz^2)    (= sin(theta) * sqrt(1-dz^2) and
  *                                                           sin(theta) / sqrt(1-dz^2))
- *   with one reciprocal r = 1 / (D (1 - dz^2)), D = 1 - g + 2gu, for both divisions:
+ *   with one reciprocal r = 1 / (D (1 - dz^2)), D = 1 - g + 2gu, for both divisions
+ *   (D formed from v = 2 - u, the float 1.k the uniform is built from):
  *   t = (1 - g^2) / D = (1 - g^2)((1 - dz^2) r) and Q = P (D r); P = x rsqrt(x)
  *   renormalisation by 1 - d/2 + 3d^2/8, d = |u|^2 - 1     (second-order rsqrt about 1;
  *                                                           |d| stays at a few ulp)
  * Returns cos(theta) for the tests. */
 float oracle_spin(oracle_photon* p, float g)
 {
-    const float u = urand(p);
+    /* u = 2 - v with v = 1.k in [1, 2) (mcml_uniform): the HG denominator 1 - g + 2gu is
+     * evaluated as (2(2g) + (1 - g)) - (2g) v, and 2u - 1 as 3 - 2v (exact) */
+    const float v = mcml_one_plus(mcml_mwc_next(&p->rng));
     const int iso = g == 0.0f;
-    const float D = iso ? 1.0f : fmaf(2.0f * g, u, 1.0f - g); /* HG denominator 1 - g + 2gu */
+    const float D = fmaf(-(2.0f * g), v, 2.0f * (2.0f * g) + (1.0f - g));
     float sinp, cosp;
     mcml_sincos_u32(mcml_mwc_next(&p->rng), mcml_sc_table, &sinp, &cosp);
     float b = fmaf(-p->dz, p->dz, 1.0f);
     float cost;
     if (b == 0.0f) { /* travelling along z */
-        cost = 2.0f * u - 1.0f;
+        cost = fmaf(-2.0f, v, 3.0f);
         if (!iso) {
             float temp = (1.0f - g * g) / D;
             cost = fmaf(-temp, temp, 1.0f + g * g) * (1.0f / (2.0f * g));
@@ -183,7 +186,7 @@ float oracle_spin(oracle_photon* p, float g)
     } else {
         /* one division serves the HG inversion (1/D = b r) and the rotation (1/b = D r) */
         float r = mcml_rcp(D * b);
-        cost = 2.0f * u - 1.0f;
+        cost = fmaf(-2.0f, v, 3.0f);
         if (!iso) {
             float temp = (1.0f - g * g) * (b * r);
             cost = fmaf(-temp, temp, 1.0f + g * g) * (1.0f / (2.0f * g));
@@ -521,9 +524,9 @@ void oracle_sincos_u32_n(const uint32_t* x, float* s, float* c, int64_t n)
 }
 void oracle_tables(const float** sc, int* n_sc, const float** lg, int* n_lg)
 {
-    *sc = mcml_sc_table;
+    *sc = &mcml_sc_table[0].a; /* pairs of floats, contiguous */
     *n_sc = MCML_SC_N;
-    *lg = mcml_lg_table;
+    *lg = &mcml_lg_table[0].a;
     *n_lg = MCML_LG_N;
 }
 void oracle_sincos_turns_n(const float* t, float* s, float* c, int64_t n)
