@@ -20,7 +20,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
+
+#include <sys/mman.h>
 
 #include "../../include/mcml.h"
 #include "mcml_device.h"
@@ -39,6 +42,7 @@ constexpr size_t kLdsTableMax = 16384;
 constexpr int kCtrlWords = 8;
 constexpr int kCtrlFirstPass = 6; /* words the first pass resets and reads back */
 constexpr int kCtrlReplayQueue = 6;
+constexpr size_t kStageBytes = 128u << 20; /* pinned staging buffer for large read-backs */
 
 struct DevBuf {
     void* p = nullptr;
@@ -68,6 +72,8 @@ struct mcml_ctx {
     DevBuf rrows, arz, a0z;           /* replay rows and absorbance grids */
     DevBuf wrows, woff, wold, wnew, wout; /* White Monte Carlo inputs and output */
     DevBuf ref_state, ref_active, ref_fibers, ref_spill_key, ref_spill_w; /* reference launch shape */
+    void* stage[2] = {nullptr, nullptr}; /* pinned read-back staging, allocated on first use */
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -122,6 +128,72 @@ void release(DevBuf& b)
 
 template <class T>
 T* as(DevBuf& b) { return static_cast<T*>(b.p); }
+
+/* Host array of a result.  Large ones are 2 MB aligned and advised for transparent huge
+ * pages, so that their first touch (the read-back) faults once per 2 MB rather than once
+ * per 4 KB.  Every result array is released with free(). */
+void* host_alloc(size_t bytes)
+{
+    if (bytes < (32u << 20)) return malloc(bytes);
+    void* p = nullptr;
+    if (posix_memalign(&p, 2u << 20, bytes) != 0) return nullptr;
+    (void)madvise(p, bytes, MADV_HUGEPAGE);
+    return p;
+}
+
+/* memcpy on up to eight host threads (pinned staging buffer -> pageable result). */
+void parallel_memcpy(char* dst, const char* src, size_t len)
+{
+    const size_t slice = 8u << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>({8, hw, (len + slice - 1) / slice});
+    if (nt <= 1) {
+        memcpy(dst, src, len);
+        return;
+    }
+    const size_t per = (len + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; t++) {
+        const size_t off = t * per;
+        if (off >= len) break;
+        th.emplace_back([=] { memcpy(dst + off, src + off, std::min(per, len - off)); });
+    }
+    memcpy(dst, src, std::min(per, len));
+    for (std::thread& t : th) t.join();
+}
+
+/* Device-to-host read-back of a result array.  Small copies go straight to `dst`; large
+ * ones run through two pinned staging buffers, the DMA of one chunk overlapping the host
+ * threads' copy of the previous one into the pageable destination. */
+int copy_d2h(mcml_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    if (bytes == 0) return MCML_OK;
+    if (bytes <= (16u << 20)) {
+        HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return MCML_OK;
+    }
+    for (int i = 0; i < 2; i++) {
+        if (!c->stage[i]) HIP_TRY(c, hipHostMalloc(&c->stage[i], kStageBytes, hipHostMallocDefault));
+        if (!c->stage_ev[i]) HIP_TRY(c, hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming));
+    }
+    const size_t n = (bytes + kStageBytes - 1) / kStageBytes;
+    auto issue = [&](size_t i) -> hipError_t {
+        const size_t off = i * kStageBytes, len = std::min(kStageBytes, bytes - off);
+        hipError_t e = hipMemcpyAsync(c->stage[i & 1], static_cast<const char*>(src) + off, len,
+                                      hipMemcpyDeviceToHost, c->stream);
+        return e != hipSuccess ? e : hipEventRecord(c->stage_ev[i & 1], c->stream);
+    };
+    HIP_TRY(c, issue(0));
+    if (n > 1) HIP_TRY(c, issue(1));
+    for (size_t i = 0; i < n; i++) {
+        HIP_TRY(c, hipEventSynchronize(c->stage_ev[i & 1]));
+        const size_t off = i * kStageBytes, len = std::min(kStageBytes, bytes - off);
+        parallel_memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(c->stage[i & 1]), len);
+        if (i + 2 < n) HIP_TRY(c, issue(i + 2)); /* into the buffer just drained */
+    }
+    return MCML_OK;
+}
 
 double ms_between(hipEvent_t a, hipEvent_t b)
 {
@@ -342,6 +414,10 @@ extern "C" void mcml_destroy(mcml_ctx* c)
                       &c->ref_state, &c->ref_active, &c->ref_fibers, &c->ref_spill_key, &c->ref_spill_w};
     for (DevBuf* b : bufs) release(*b);
     mcml_graph_release(c->graph);
+    for (int i = 0; i < 2; i++) {
+        if (c->stage[i]) (void)hipHostFree(c->stage[i]);
+        if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
+    }
     if (c->d_mult) (void)hipFree(c->d_mult);
     if (c->ctrl) (void)hipFree(c->ctrl);
     if (c->h_ctrl) (void)hipHostFree(c->h_ctrl);
@@ -800,20 +876,24 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
             }
     out->n_records = n;
     if (!o.skip_records) {
-        out->records = (mcml_record*)malloc(sizeof(mcml_record) * (size_t)std::max<uint64_t>(n, 1));
+        out->records = (mcml_record*)host_alloc(sizeof(mcml_record) * (size_t)std::max<uint64_t>(n, 1));
         if (!out->records) {
             mcml_result_free(out);
             return MCML_ERR_NOMEM;
         }
         if (n)
-            HIP_TRY(c, hipMemcpy(out->records, c->precs.p, sizeof(mcml_record) * (size_t)n, hipMemcpyDeviceToHost));
+            rc = copy_d2h(c, out->records, c->precs.p, sizeof(mcml_record) * (size_t)n);
+        if (rc) {
+            mcml_result_free(out);
+            return rc;
+        }
     }
     const double t1 = now_ms();
     out->ms_first_pass_wall = t1 - t0;
     if (o.replay) {
         out->pl_offset = (uint64_t*)malloc(sizeof(uint64_t) * (groups + 1));
         out->replay_reflectance = (double*)calloc(groups, sizeof(double));
-        out->pathlength = (float*)malloc(sizeof(float) * (size_t)std::max<uint64_t>(n, 1) * (size_t)(L + 2));
+        out->pathlength = (float*)host_alloc(sizeof(float) * (size_t)std::max<uint64_t>(n, 1) * (size_t)(L + 2));
         if (!out->pl_offset || !out->replay_reflectance || !out->pathlength) {
             mcml_result_free(out);
             return MCML_ERR_NOMEM;
@@ -822,9 +902,10 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
         if (inline_pl) { /* the first pass already produced the replay rows, in file order */
             memcpy(out->replay_reflectance, out->reflectance, sizeof(double) * groups);
             out->ms_dump_kernel = dump_ms;
-            if (n)
-                HIP_TRY(c, hipMemcpy(out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2),
-                                     hipMemcpyDeviceToHost));
+            if ((rc = copy_d2h(c, out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2)))) {
+                mcml_result_free(out);
+                return rc;
+            }
         } else { /* absorbance maps need the true replay (transport.cu:160-222) */
             const bool grids = o.absorbance != 0;
             const size_t rz_cells = grids ? groups * (size_t)nz * (size_t)nr : 0;
@@ -878,18 +959,22 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
                 mcml_result_free(out);
                 return rc;
             }
-            if (n)
-                HIP_TRY(c, hipMemcpy(out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2),
-                                     hipMemcpyDeviceToHost));
+            if ((rc = copy_d2h(c, out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2)))) {
+                mcml_result_free(out);
+                return rc;
+            }
             if (grids) {
-                out->A_rz = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(rz_cells, 1));
+                out->A_rz = (uint64_t*)host_alloc(sizeof(uint64_t) * std::max<size_t>(rz_cells, 1));
                 out->A0_z = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(z_cells, 1));
                 if (!out->A_rz || !out->A0_z) {
                     mcml_result_free(out);
                     return MCML_ERR_NOMEM;
                 }
-                HIP_TRY(c, hipMemcpy(out->A_rz, c->arz.p, sizeof(uint64_t) * rz_cells, hipMemcpyDeviceToHost));
-                HIP_TRY(c, hipMemcpy(out->A0_z, c->a0z.p, sizeof(uint64_t) * z_cells, hipMemcpyDeviceToHost));
+                if ((rc = copy_d2h(c, out->A_rz, c->arz.p, sizeof(uint64_t) * rz_cells)) ||
+                    (rc = copy_d2h(c, out->A0_z, c->a0z.p, sizeof(uint64_t) * z_cells))) {
+                    mcml_result_free(out);
+                    return rc;
+                }
             }
         }
         out->replayed = 1;
