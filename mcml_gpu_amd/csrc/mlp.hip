@@ -33,41 +33,28 @@ __device__ __forceinline__ short to_bf16(float x)
     return (short)(u >> 16);
 }
 
-/* Standard normal CDF Phi(z) = (1 + erf(z / sqrt 2)) / 2 with erf from Abramowitz & Stegun
- * 7.1.26 (|error| <= 1.5e-7, far below the bf16 rounding the activations then get). */
-__device__ __forceinline__ float normal_cdf(float z)
-{
-    const float x = fabsf(z) * 0.70710678118654752f;
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, x, 1.0f));
-    float p = fmaf(1.061405429f, t, -1.453152027f);
-    p = fmaf(p, t, 1.421413741f);
-    p = fmaf(p, t, -0.284496736f);
-    p = fmaf(p, t, 0.254829592f);
-    p *= t;
-    const float erf_abs = 1.0f - p * __expf(-x * x);
-    return 0.5f + 0.5f * copysignf(erf_abs, z);
-}
+/* GELU in its tanh form (Hendrycks & Gimpel, "Gaussian Error Linear Units", 2016; PyTorch's
+ * gelu(approximate="tanh")): 0.5 z (1 + tanh(k (z + c z^3))) = z sigmoid(a),
+ * a = 2k (z + c z^3), k = sqrt(2/pi), c = 0.044715.  The sigmoid is 1 / (1 + 2^(-a log2 e))
+ * with the hardware exp2 and reciprocal (a few ulp, far below the bf16 rounding the
+ * activations then get), log2(e) folded into the constants: 2^(z (GELU_A + GELU_B z^2)). */
+constexpr float GELU_A = -2.3022082f;   /* -2k log2(e) */
+constexpr float GELU_B = -0.10294324f;  /* -2kc log2(e) */
+constexpr float GELU_K2 = 1.5957691f;   /* 2k: a'(z) = 2k + 6kc z^2 */
+constexpr float GELU_K6C = 0.21406445f; /* 6kc */
 
-/* GELU(z) = z Phi(z) with the same erf approximation (A&S 7.1.26), rearranged for fewer
- * operations: z (1 + sign(z) erf(|z|/sqrt 2)) / 2 = max(z, 0) - |z| (poly(t) / 2) e^{-z^2/2},
- * t = 1 / (1 + 0.3275911 |z| / sqrt 2); the 1/2 and 1/sqrt 2 are folded into the constants
- * and e^{-z^2/2} = 2^(z (z (-log2(e) / 2))). */
 __device__ __forceinline__ float gelu(float z)
 {
-    const float az = fabsf(z);
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f * 0.70710678118654752f, az, 1.0f));
-    float p = fmaf(0.5f * 1.061405429f, t, 0.5f * -1.453152027f);
-    p = fmaf(p, t, 0.5f * 1.421413741f);
-    p = fmaf(p, t, 0.5f * -0.284496736f);
-    p = fmaf(p, t, 0.5f * 0.254829592f);
-    p *= t;
-    const float e = __builtin_amdgcn_exp2f(z * (z * -0.72134752044448170f));
-    return fmaf(-az, p * e, fmaxf(z, 0.0f));
+    const float e = __builtin_amdgcn_exp2f(z * fmaf(z * z, GELU_B, GELU_A));
+    return z * __builtin_amdgcn_rcpf(1.0f + e);
 }
 
+/* d/dz z s(a) = s + z s (1 - s) a'(z), s = sigmoid(a) */
 __device__ __forceinline__ float gelu_grad(float z)
 {
-    return normal_cdf(z) + z * (0.39894228040143268f * __expf(-0.5f * z * z));
+    const float z2 = z * z;
+    const float s = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(z2, GELU_B, GELU_A)));
+    return fmaf(z * s * (1.0f - s), fmaf(z2, GELU_K6C, GELU_K2), s);
 }
 
 } // namespace

@@ -153,12 +153,16 @@ void parallel_memcpy(char* dst, const char* src, size_t len)
     }
     const size_t per = (len + nt - 1) / nt;
     std::vector<std::thread> th;
-    for (size_t t = 1; t < nt; t++) {
-        const size_t off = t * per;
-        if (off >= len) break;
-        th.emplace_back([=] { memcpy(dst + off, src + off, std::min(per, len - off)); });
+    size_t done = per; /* slices [0, done) are copied by this thread or a started one */
+    try {
+        for (size_t t = 1; t < nt && done < len; t++, done += per) {
+            const size_t off = t * per;
+            th.emplace_back([=] { memcpy(dst + off, src + off, std::min(per, len - off)); });
+        }
+    } catch (...) { /* no thread to spare: this thread copies the rest */
     }
     memcpy(dst, src, std::min(per, len));
+    if (done < len) memcpy(dst + done, src + done, len - done);
     for (std::thread& t : th) t.join();
 }
 

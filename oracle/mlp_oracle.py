@@ -6,7 +6,6 @@ exactly as the matrix-core kernels round them (round to nearest even).  With bf1
 is the plain float64 network, whose gradient the tests check against finite differences.
 """
 import numpy as np
-from scipy.special import erf
 
 
 def to_bf16(x):
@@ -16,12 +15,18 @@ def to_bf16(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+GELU_K = np.sqrt(2.0 / np.pi)
+GELU_C = 0.044715
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    """GELU in its tanh form: 0.5 z (1 + tanh(k (z + c z^3))), k = sqrt(2/pi), c = 0.044715."""
+    return 0.5 * z * (1.0 + np.tanh(GELU_K * z * (1.0 + GELU_C * z * z)))
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+    t = np.tanh(GELU_K * z * (1.0 + GELU_C * z * z))
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * GELU_K * (1.0 + 3.0 * GELU_C * z * z)
 
 
 def unpack(params, widths):

@@ -42,7 +42,7 @@ def torch_net(params, widths):
         for i, (W, b) in enumerate(zip(Ws, bs)):
             a = a @ W.T + b
             if i + 1 < len(Ws):
-                a = torch.nn.functional.gelu(a)
+                a = torch.nn.functional.gelu(a, approximate="tanh")
         return a
     return run, Ws, bs
 
@@ -61,11 +61,13 @@ def test_bf16_rounding_matches_torch():
 
 
 def test_gelu_and_derivative_match_torch():
+    """The tanh form of GELU (torch's approximate="tanh") and its derivative."""
     z = torch.linspace(-8, 8, 4001, dtype=torch.float64, requires_grad=True)
-    g = torch.nn.functional.gelu(z)
+    g = torch.nn.functional.gelu(z, approximate="tanh")
     (dg,) = torch.autograd.grad(g.sum(), z)
     assert np.allclose(MO.gelu(z.detach().numpy()), g.detach().numpy(), rtol=1e-14, atol=1e-15)
-    assert np.allclose(MO.gelu_grad(z.detach().numpy()), dg.numpy(), rtol=1e-13, atol=1e-15)
+    # atol: in saturation torch's tanh reaches -1 exactly while 0.5 (1 + t) + ... leaves ~5e-15
+    assert np.allclose(MO.gelu_grad(z.detach().numpy()), dg.numpy(), rtol=1e-13, atol=1e-14)
 
 
 @pytest.mark.parametrize("widths", [SMALL, ODD])
