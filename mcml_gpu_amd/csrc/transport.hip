@@ -547,7 +547,6 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
      * 10 000 steps, far below 2^32 */
     uint32_t c_photons = 0, c_steps = 0, c_bnd = 0, c_int = 0;
     uint32_t n_int = 0;
-    GridLane gl = {nullptr, nullptr, nullptr, ~0ull, 0, 0, ~0u};
     const int refill_min = a.refill_min < 1 ? 1 : a.refill_min;
     const int bnd_min = a.bnd_min < 1 ? 1 : a.bnd_min;
 
@@ -580,16 +579,24 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
          * or ends its photon sits out the rest of the block */
 #pragma unroll
         for (int kk = 0; kk < STEPS_PER_CHECK; kk++) {
-            if (alive && !pending && st == ST_ALIVE) {
-                float s;
-                int nl;
-                if (hop<PL>(p, s, nl, pl_lane, pl_stride)) {
-                    pending = true;
-                    pend_layer = nl;
-                } else {
-                    st = interact<false>(p, s, gl, nullptr, n_int);
-                }
+            /* interact<false> written as flat conditions: one level of divergence around
+             * the spin instead of three, so the compiler keeps the direction in place
+             * rather than copying it in and out of each level */
+            const bool act = alive && !pending && st == ST_ALIVE;
+            float s = 0.0f;
+            int nl = 0;
+            bool bnd = false;
+            if (act) bnd = hop<PL>(p, s, nl, pl_lane, pl_stride);
+            if (act && bnd) {
+                pending = true;
+                pend_layer = nl;
             }
+            if (act && !bnd && s > 0.0f) {
+                n_int++;
+                p.w -= rintf(p.albedo * p.w);
+                spin(p);
+            }
+            if (act && !bnd) st = survive(p);
         }
         if (resolve_now(pending, alive, bnd_min) && pending) {
             pending = false;
