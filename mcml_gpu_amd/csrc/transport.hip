@@ -709,19 +709,24 @@ extern "C" __global__ void __launch_bounds__(256) mcml_refshape_pass(RefShapeArg
         Photon p = slot->p;
         uint64_t k = slot->k;
         const DevSet* S = a.set;
-        GridLane gl = {nullptr, nullptr, nullptr, ~0ull, 0, 0, ~0u};
         bool live = true;
         for (int ii = 0; ii < a.steps; ii++) {
             float s;
             int nl;
             uint32_t mask = 0;
-            int st;
-            if (hop<false>(p, s, nl, nullptr, 0)) {
+            int st = ST_ALIVE;
+            /* the persistent kernel's step, flat conditions included */
+            const bool bnd = hop<false>(p, s, nl, nullptr, 0);
+            if (bnd) {
                 c_bnd++;
                 st = cross<false>(p, nl, S, 0, mask, nullptr, 0);
-            } else {
-                st = interact<false>(p, s, gl, nullptr, n_int);
             }
+            if (!bnd && s > 0.0f) {
+                n_int++;
+                p.w -= rintf(p.albedo * p.w);
+                spin(p);
+            }
+            if (!bnd) st = survive(p);
             if (st == ST_ALIVE) continue;
             c_steps += (uint32_t)p.steps;
             if (st == ST_DETECTED) { /* detect(), transport.cu:872-938 */
