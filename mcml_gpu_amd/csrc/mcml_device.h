@@ -85,7 +85,11 @@ struct ReplayArgs {
     uint64_t njobs;
     uint64_t seed;
     const uint32_t* mult;
-    unsigned long long* queue;
+    /* jobs of (set, SDS) group g are [grp_begin[g], grp_begin[g+1]); grp_queue[g] counts
+     * the jobs of group g handed out so far (zero at launch) */
+    const uint64_t* grp_begin;
+    unsigned long long* grp_queue;
+    int32_t ngroups;
     int32_t chunk;
     int32_t num_layers;
     int32_t bnd_min;

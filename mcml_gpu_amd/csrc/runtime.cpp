@@ -41,7 +41,6 @@ constexpr int kDefaultBoundary = 4;
 constexpr size_t kLdsTableMax = 16384;
 constexpr int kCtrlWords = 8;
 constexpr int kCtrlFirstPass = 6; /* words the first pass resets and reads back */
-constexpr int kCtrlReplayQueue = 6;
 constexpr size_t kStageBytes = 128u << 20; /* pinned staging buffer for large read-backs */
 
 struct DevBuf {
@@ -60,7 +59,7 @@ struct mcml_ctx {
     char err[1024] = {0};
     uint32_t* d_mult = nullptr;
     /* control block: [0] first-pass ticket counter, [1] records appended, [2..5] work
-     * counters, [6] replay job counter; mirrored into pinned host memory */
+     * counters; mirrored into pinned host memory */
     unsigned long long* ctrl = nullptr;
     unsigned long long* h_ctrl = nullptr;
     FirstPassGraph graph;
@@ -70,6 +69,7 @@ struct mcml_ctx {
     DevBuf rows, weights, precs;      /* sorted rows (file image), weights, public records */
     DevBuf offsets, sums;             /* per (set, SDS) group: start offsets, FP64 sums */
     DevBuf rrows, arz, a0z;           /* replay rows and absorbance grids */
+    DevBuf rgrp, rgq;                 /* replay: job range and hand-out counter per (set, SDS) */
     DevBuf wrows, woff, wold, wnew, wout; /* White Monte Carlo inputs and output */
     DevBuf ref_state, ref_active, ref_fibers, ref_spill_key, ref_spill_w; /* reference launch shape */
     void* stage[2] = {nullptr, nullptr}; /* pinned read-back staging, allocated on first use */
@@ -415,7 +415,8 @@ extern "C" void mcml_destroy(mcml_ctx* c)
     DevBuf* bufs[] = {&c->sets,  &c->set_begin, &c->rec_key, &c->rec_row, &c->skey,  &c->sidx, &c->stemp,
                       &c->rows,  &c->weights,   &c->precs,   &c->offsets, &c->sums,  &c->rrows, &c->arz,
                       &c->a0z,   &c->wrows,     &c->woff,    &c->wold,    &c->wnew,  &c->wout,
-                      &c->ref_state, &c->ref_active, &c->ref_fibers, &c->ref_spill_key, &c->ref_spill_w};
+                      &c->ref_state, &c->ref_active, &c->ref_fibers, &c->ref_spill_key, &c->ref_spill_w,
+                      &c->rgrp,      &c->rgq};
     for (DevBuf* b : bufs) release(*b);
     mcml_graph_release(c->graph);
     for (int i = 0; i < 2; i++) {
@@ -925,7 +926,14 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
                 HIP_TRY(c, hipMemsetAsync(c->a0z.p, 0, sizeof(unsigned long long) * z_cells, c->stream));
             }
             if (n) {
-                HIP_TRY(c, hipMemsetAsync(c->ctrl + kCtrlReplayQueue, 0, sizeof(unsigned long long), c->stream));
+                if ((rc = ensure(c, c->rgrp, sizeof(uint64_t) * (groups + 1))) ||
+                    (rc = ensure(c, c->rgq, sizeof(unsigned long long) * groups))) {
+                    mcml_result_free(out);
+                    return rc;
+                }
+                HIP_TRY(c, hipMemcpyAsync(c->rgrp.p, off.data(), sizeof(uint64_t) * (groups + 1),
+                                          hipMemcpyHostToDevice, c->stream));
+                HIP_TRY(c, hipMemsetAsync(c->rgq.p, 0, sizeof(unsigned long long) * groups, c->stream));
                 ReplayArgs a;
                 memset(&a, 0, sizeof(a));
                 a.sets = as<DevSet>(c->sets);
@@ -933,7 +941,9 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
                 a.njobs = n;
                 a.seed = seed;
                 a.mult = c->d_mult;
-                a.queue = c->ctrl + kCtrlReplayQueue;
+                a.grp_begin = as<uint64_t>(c->rgrp);
+                a.grp_queue = as<unsigned long long>(c->rgq);
+                a.ngroups = (int32_t)groups;
                 a.chunk = 64;
                 a.num_layers = L;
                 a.bnd_min = o.boundary_min > 0 ? std::min(o.boundary_min, 64) : kDefaultBoundary;

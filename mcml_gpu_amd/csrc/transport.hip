@@ -773,6 +773,54 @@ extern "C" __global__ void __launch_bounds__(256) mcml_refshape_pass(RefShapeArg
 /* ----------------------------------------------------------------- replay */
 #define REPLAY_BLOCK 256
 
+/* Replay job queue of one wavefront.  Jobs are claimed in chunks from per-(set, SDS)
+ * group counters, so a wavefront stays with one group -- and with the group its LDS
+ * absorbance tile holds -- until that group runs dry, then moves to the next group with
+ * jobs left.  (One counter over all jobs hands a wavefront chunks from wherever the sweep
+ * has reached: with a few jobs per lane its lanes soon hold several groups, and the
+ * deposits of all but one go to global atomics.) */
+struct JobQueue {
+    uint64_t next, end; /* the wavefront's claimed, not yet handed out, job range */
+    int32_t g, g_start; /* current group; the group it started from */
+    bool dry;
+};
+
+__device__ __forceinline__ uint64_t take_jobs(JobQueue& q, uint64_t need, bool mine, const ReplayArgs& a)
+{
+    const uint64_t n = (uint64_t)__popcll(need);
+    const uint64_t rank = rank_in(need);
+    const uint64_t avail = q.end - q.next;
+    uint64_t id = ~0ull;
+    if (mine && rank < avail) id = q.next + rank;
+    if (n <= avail) {
+        q.next += n;
+        return id;
+    }
+    q.next = q.end;
+    const uint32_t first = (uint32_t)(__ffsll((long long)__ballot(1)) - 1);
+    while (!q.dry) { /* wave-uniform */
+        const uint64_t gb = a.grp_begin[q.g], ge = a.grp_begin[q.g + 1];
+        unsigned long long base = 0;
+        if (lane_id() == first) base = atomicAdd(&a.grp_queue[q.g], (unsigned long long)a.chunk);
+        base = bcast_first(base);
+        if (gb + base < ge) {
+            q.next = gb + base;
+            q.end = gb + base + (uint64_t)a.chunk < ge ? gb + base + (uint64_t)a.chunk : ge;
+            break;
+        }
+        q.g = q.g + 1 == a.ngroups ? 0 : q.g + 1;
+        if (q.g == q.g_start) q.dry = true;
+    }
+    if (q.dry) return id;
+    if (mine && rank >= avail) {
+        const uint64_t t = q.next + (rank - avail);
+        if (t < q.end) id = t;
+    }
+    const uint64_t take = n - avail, left = q.end - q.next;
+    q.next += take < left ? take : left;
+    return id;
+}
+
 /* Add a wavefront's tile into its group's global grids and clear it (wave-cooperative). */
 __device__ __forceinline__ void flush_tile(unsigned long long* tile, uint32_t grp, const ReplayArgs& a)
 {
@@ -800,7 +848,6 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
     float* pl_lane = pl_lds + threadIdx.x;
     const int L = a.num_layers;
     const int bnd_min = a.bnd_min < 1 ? 1 : a.bnd_min;
-    WaveQueue q = {0, 0, false};
     Photon p;
     bool alive = false, pending = false;
     int pend_layer = 0;
@@ -808,6 +855,19 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
     uint32_t jset = 0, jsds = 1;
     const DevSet* S = a.sets;
     GridLane gl = {nullptr, nullptr, nullptr, ~0ull, 0, 0, ~0u};
+    JobQueue jq = {0, 0, 0, 0, a.ngroups < 1};
+    if (!jq.dry) { /* start in the group holding this wavefront's share of the job order */
+        const uint64_t w = ((uint64_t)blockIdx.x * REPLAY_BLOCK + threadIdx.x) >> 6;
+        const uint64_t nw = ((uint64_t)gridDim.x * REPLAY_BLOCK) >> 6;
+        const uint64_t j = w * a.njobs / nw;
+        int lo = 0, hi = a.ngroups;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.grp_begin[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        jq.g = jq.g_start = lo;
+    }
     if (a.A_rz != nullptr) {
         gl.tile = grid_tiles[threadIdx.x >> 6];
         for (int i = lane_id(); i < TILE_WORDS; i += 64) gl.tile[i] = 0ull;
@@ -816,9 +876,8 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
 
     for (;;) {
         const uint64_t idle = __ballot(!alive);
-        if (!q.dry && idle != 0) {
-            uint64_t claimed;
-            const uint64_t t = take_ids(q, idle, !alive, a.queue, (uint64_t)a.chunk, a.njobs, claimed);
+        if (!jq.dry && idle != 0) {
+            const uint64_t t = take_jobs(jq, idle, !alive, a);
             if (t != ~0ull) {
                 job = t;
                 const uint64_t key = a.keys[job];
@@ -836,12 +895,21 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
                 }
                 alive = true;
             }
-            /* wave-uniform: once no live lane uses the tile's group, hand the tile to the
-             * group of the first live lane */
-            if (gl.tile != nullptr && __ballot(alive && gl.grp == gl.tile_grp) == 0 && __ballot(alive) != 0) {
-                flush_tile(gl.tile, gl.tile_grp, a);
+            /* wave-uniform: when the tile's group holds under a quarter of the live lanes,
+             * hand the tile to the group of the first other live lane if that group holds
+             * more (a wavefront moving on to a new group keeps a few long photons of the
+             * old one for a while) */
+            if (gl.tile != nullptr) {
                 const uint64_t live = __ballot(alive);
-                gl.tile_grp = (uint32_t)__shfl((int)gl.grp, __ffsll((long long)live) - 1, 64);
+                const uint64_t held = __ballot(alive && gl.grp == gl.tile_grp);
+                if (live != 0 && 4 * __popcll(held) < __popcll(live)) {
+                    const uint64_t others = live & ~held;
+                    const uint32_t cand = (uint32_t)__shfl((int)gl.grp, __ffsll((long long)others) - 1, 64);
+                    if (__popcll(__ballot(alive && gl.grp == cand)) > __popcll(held)) {
+                        flush_tile(gl.tile, gl.tile_grp, a);
+                        gl.tile_grp = cand;
+                    }
+                }
             }
         }
         if (__ballot(alive) == 0) break;
