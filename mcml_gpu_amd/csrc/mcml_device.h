@@ -99,7 +99,7 @@ struct ReplayArgs {
     unsigned long long* A_rz;
     unsigned long long* A0_z;
     int32_t num_sds, nr, nz, pad;
-    float dr, dz;
+    float rdr, rdz;            /* 1/dr, 1/dz: bins are trunc(r * rdr), trunc(z * rdz) */
 };
 
 /* Reference launch shape (transport.cu:57-147, 341-471): one photon per thread, at most

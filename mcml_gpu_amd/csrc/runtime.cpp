@@ -953,8 +953,8 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
                 a.num_sds = S;
                 a.nr = nr;
                 a.nz = nz;
-                a.dr = der[0].grid_dr;
-                a.dz = der[0].grid_dz;
+                a.rdr = 1.0f / der[0].grid_dr;
+                a.rdz = 1.0f / der[0].grid_dz;
                 int occ = 0;
                 HIP_TRY(c, mcml_occupancy_replay(&occ));
                 if (occ < 1) occ = 1;

@@ -376,7 +376,7 @@ __device__ __forceinline__ int interact(Photon& p, float s, GridLane& gl, const 
         const float w_temp = rintf(p.albedo * p.w);
         p.w -= w_temp;
         if (REPLAY && gl.a_rz != nullptr) {
-            float qz = p.z / ra->dz;
+            float qz = p.z * ra->rdz;
             qz = fminf(fmaxf(qz, 0.0f), (float)(ra->nz - 1));
             const int zb = (int)qz;
             if (p.first) {
@@ -385,7 +385,7 @@ __device__ __forceinline__ int interact(Photon& p, float s, GridLane& gl, const 
                 else
                     atomicAdd(&gl.a0_z[zb], (unsigned long long)w_temp);
             } else {
-                float qr = sqrtf(p.x * p.x + p.y * p.y) / ra->dr;
+                float qr = sqrtf(p.x * p.x + p.y * p.y) * ra->rdr;
                 qr = fminf(fmaxf(qr, 0.0f), (float)(ra->nr - 1));
                 const uint64_t key = ((uint64_t)(uint32_t)zb << 32) | (uint32_t)(int)qr;
                 if (key == gl.rl_key) {

@@ -305,16 +305,18 @@ uint32_t oracle_detect(const mcml_derived* d, const oracle_photon* p, int only, 
     return mask;
 }
 
-/* Bins of the absorbance grid, transport.cu:583, 588 (trunc then clamp to the last bin). */
+/* Bins of the absorbance grid, transport.cu:583, 588 (trunc then clamp to the last bin).
+ * The reference's __fdividef(z, dz) is a product with a reciprocal; here the reciprocal is
+ * the correctly rounded 1/dz. */
 static int zbin_of(const mcml_derived* d, float z)
 {
-    float q = z / d->grid_dz;
+    float q = z * (1.0f / d->grid_dz);
     q = fminf(fmaxf(q, 0.0f), (float)(d->grid_nz - 1));
     return (int)q;
 }
 static int rbin_of(const mcml_derived* d, float x, float y)
 {
-    float q = sqrtf(x * x + y * y) / d->grid_dr;
+    float q = sqrtf(x * x + y * y) * (1.0f / d->grid_dr);
     q = fminf(fmaxf(q, 0.0f), (float)(d->grid_nr - 1));
     return (int)q;
 }
