@@ -915,15 +915,18 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
         if (__ballot(alive) == 0) break;
         int st = ST_ALIVE;
         uint32_t mask = 0;
-        if (alive && !pending) {
-            float s;
-            int nl;
-            if (hop<true>(p, s, nl, pl_lane, REPLAY_BLOCK)) {
+#pragma unroll
+        for (int kk = 0; kk < STEPS_PER_CHECK; kk++) { /* as in the first pass */
+            const bool act = alive && !pending && st == ST_ALIVE;
+            float s = 0.0f;
+            int nl = 0;
+            bool bnd = false;
+            if (act) bnd = hop<true>(p, s, nl, pl_lane, REPLAY_BLOCK);
+            if (act && bnd) {
                 pending = true;
                 pend_layer = nl;
-            } else {
-                st = interact<true>(p, s, gl, &a, n_int);
             }
+            if (act && !bnd) st = interact<true>(p, s, gl, &a, n_int);
         }
         if (resolve_now(pending, alive, bnd_min) && pending) {
             pending = false;
