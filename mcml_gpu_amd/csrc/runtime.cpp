@@ -595,10 +595,11 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
  * when set, so that tests can force many relaunches) until no thread
  * is active; after each launch the host clears and uploads the per-thread detection
  * buffers, reads back the active flags and the buffers and reduces them (fiber.cu:3-13,
- * transport.cu:299-311).  The records end in c->rec_key / c->rec_row like first_pass's,
- * and host_count holds the per-detector counts of the host reduction. */
+ * transport.cu:299-311).  The records end in c->rec_key / c->rec_row like first_pass's;
+ * host_count and host_sum hold the per-detector counts and weight sums of the host
+ * reduction. */
 int first_pass_refshape(mcml_ctx* c, const mcml_derived& d, const mcml_opts& o, uint64_t seed, mcml_result* out,
-                        uint64_t& n_rec, std::vector<uint64_t>& host_count)
+                        uint64_t& n_rec, std::vector<uint64_t>& host_count, std::vector<double>& host_sum)
 {
     const int S = d.num_sds;
     DevSet hs;
@@ -628,7 +629,8 @@ int first_pass_refshape(mcml_ctx* c, const mcml_derived& d, const mcml_opts& o, 
     std::vector<uint32_t> hact((size_t)T);
     std::vector<uint64_t> hkey;
     std::vector<float> hw;
-    std::vector<double> hsum((size_t)S);
+    std::vector<double>& hsum = host_sum;
+    hsum.assign((size_t)S, 0.0);
     double ms_kernel = 0;
     int launches = 0;
     uint64_t n_spill = 0;
@@ -729,9 +731,16 @@ int first_pass_refshape(mcml_ctx* c, const mcml_derived& d, const mcml_opts& o, 
         HIP_TRY(c, hipMemcpyAsync(sk.data(), c->ref_spill_key.p, sizeof(uint64_t) * (size_t)n_spill,
                                   hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
-        for (uint64_t key : sk) {
-            const uint64_t s = (key >> MCML_KEY_SDS_SHIFT) & 31u;
-            if (s >= 1 && s <= (uint64_t)S) host_count[(size_t)s - 1]++;
+        std::vector<float> sw((size_t)n_spill);
+        HIP_TRY(c, hipMemcpyAsync(sw.data(), c->ref_spill_w.p, sizeof(float) * (size_t)n_spill, hipMemcpyDeviceToHost,
+                                  c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < sk.size(); i++) {
+            const uint64_t s = (sk[i] >> MCML_KEY_SDS_SHIFT) & 31u;
+            if (s >= 1 && s <= (uint64_t)S) {
+                host_count[(size_t)s - 1]++;
+                hsum[(size_t)s - 1] += (double)sw[i];
+            }
         }
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -861,8 +870,9 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
     const double t0 = now_ms();
     uint64_t n = 0;
     std::vector<uint64_t> off, host_count;
+    std::vector<double> host_sum;
     double dump_ms = 0;
-    rc = o.reference_shape ? first_pass_refshape(c, der[0], o, seed, out, n, host_count)
+    rc = o.reference_shape ? first_pass_refshape(c, der[0], o, seed, out, n, host_count, host_sum)
                            : first_pass(c, der, o, seed, stride, out, n);
     if (rc || (rc = sort_records(c, n)) ||
         (rc = gather_and_reduce(c, n, nsets, S, as<float>(c->rec_row), stride, as<uint32_t>(c->sidx),
@@ -871,11 +881,13 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
         return rc;
     }
     for (size_t g = 0; g < groups; g++) out->detected[g] = off[g + 1] - off[g];
-    if (o.reference_shape)
+    if (o.reference_shape) /* the host reduction (calculate_reflectance) agrees with the device's */
         for (size_t g = 0; g < groups; g++)
-            if (out->detected[g] != host_count[g]) {
-                set_err(c, "reference launch shape: SDS %d host reduction counted %llu records, the device %llu",
-                        (int)g + 1, (unsigned long long)host_count[g], (unsigned long long)out->detected[g]);
+            if (out->detected[g] != host_count[g] ||
+                fabs(out->reflectance[g] - host_sum[g]) > 1e-9 * fabs(out->reflectance[g])) {
+                set_err(c, "reference launch shape: SDS %d host reduction %llu records / %.17g, the device %llu / %.17g",
+                        (int)g + 1, (unsigned long long)host_count[g], host_sum[g],
+                        (unsigned long long)out->detected[g], out->reflectance[g]);
                 mcml_result_free(out);
                 return MCML_ERR_DEVICE;
             }
