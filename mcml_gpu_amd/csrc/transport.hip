@@ -503,8 +503,9 @@ __device__ __forceinline__ bool resolve_now(bool pending, bool alive, int bnd_mi
 
 /* Photon steps a wavefront takes between its refill and boundary decisions.  Each decision
  * costs a few ballots and selects; making them every sixth step instead of every step
- * took the cfg1 first pass from 215 to 197 ms (3: 200, 4: 199). */
-#define STEPS_PER_CHECK 6
+ * took the cfg1 first pass from 215 to 197 ms (3: 200, 4: 199); with the step as flat
+ * conditions, twelve gain another 1 % over six (8: 0.5 %). */
+#define STEPS_PER_CHECK 12
 
 /* Set owning ticket t, using the wavefront's cached set range [clo, chi) first. */
 __device__ __forceinline__ uint32_t set_for(const uint64_t* set_begin, int nsets, uint64_t t, uint32_t cset,
