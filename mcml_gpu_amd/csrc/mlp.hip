@@ -213,18 +213,23 @@ constexpr int FW_MB = 2;                /* 16-row blocks per wavefront */
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-/* Copy N 16-byte words of packed weights into LDS: every load is issued before the first
- * store, so the copy costs one memory latency instead of one per word. */
+/* Copy N 16-byte words of packed weights into LDS with direct global-to-LDS loads
+ * (global_load_lds_dwordx4: lane i of a wavefront writes the 16 bytes it loads to the
+ * wavefront's LDS base + 16 i).  No VGPRs hold the data, so all N / T loads of a lane are
+ * in flight together; the barrier that follows waits for them.  (Through registers, with
+ * the activations of the layer occupying most of them, the compiler issued one load,
+ * waited for it and stored it, sixteen times in a row per hidden layer.) */
 template <int N>
 __device__ __forceinline__ void stage(uint4* lds, const uint4* g)
 {
     constexpr int T = MLP_FUSED_WAVES * 64, PER = N / T;
     static_assert(N % T == 0, "stage size must be a multiple of the workgroup");
-    uint4 v[PER];
+    const int wave_base = threadIdx.x & ~63;
 #pragma unroll
-    for (int i = 0; i < PER; i++) v[i] = g[threadIdx.x + i * T];
-#pragma unroll
-    for (int i = 0; i < PER; i++) lds[threadIdx.x + i * T] = v[i];
+    for (int i = 0; i < PER; i++)
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(g + threadIdx.x + i * T),
+                                         (__attribute__((address_space(3))) void*)(lds + i * T + wave_base), 16, 0,
+                                         0);
 }
 
 /* One GELU layer from KT_IN k steps of input to MLP_FUSED_H outputs. */
