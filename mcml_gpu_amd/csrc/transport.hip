@@ -544,8 +544,9 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
     WaveQueue q = {0, 0, false};
     uint32_t cset = 0;
     uint64_t clo = 0, chi = 0;
-    /* per-lane work counters; a lane runs at most a few thousand photons of at most
-     * 10 000 steps, far below 2^32 */
+    /* per-lane work counters (statistics only; the results do not depend on them).  A
+     * lane's steps run one after another, about a microsecond each at full occupancy, so
+     * its totals stay below 2^32 unless one launch runs for more than an hour. */
     uint32_t c_photons = 0, c_steps = 0, c_bnd = 0, c_int = 0;
     uint32_t n_int = 0;
     const int refill_min = a.refill_min < 1 ? 1 : a.refill_min;
