@@ -7,11 +7,10 @@ Monte Carlo (mcml_run_batch) of the cfg3 parameter sets, expanded over absorptio
 White Monte Carlo rescaling of the recorded pathlengths (mcml_white_rescale).
 """
 import ctypes as C
-import time
 
 import numpy as np
 
-from . import Context, MCMLError, abi, batch, load_library, make_opts
+from . import MCMLError, batch, load_library, make_opts
 
 WIDTHS = (14, 256, 256, 256, 256, 3)
 W_UNIT = 4294967295.0  # reflectance normaliser per photon (io.cpp:16)

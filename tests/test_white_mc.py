@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle_ffi as O
-from mcml_gpu_amd import abi, batch, make_opts
+from mcml_gpu_amd import batch, make_opts
 
 
 def white_numpy(rows, offsets, mua_old, mua_new, power=1):
