@@ -287,7 +287,7 @@ enum StepOutcome { ST_ALIVE = 0, ST_DEAD = 1, ST_DETECTED = 2 };
 /* Hop (transport.cu:372-395): sample the step, stop it at the layer boundary, move, count
  * the step, clamp z.  Returns true when the hop ended on a boundary (*nl = layer beyond). */
 template <bool REPLAY>
-__device__ __forceinline__ bool hop(Photon& p, float& s, int& nl, float* pl_lane, int pl_stride)
+__device__ __forceinline__ bool hop(Photon& p, float& s, int& nl)
 {
     s = mcml_neglog_u32(mcml_mwc_next(&p.rng), LG_TAB) * p.mutr;
     nl = p.layer;
@@ -588,7 +588,7 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
             float s = 0.0f;
             int nl = 0;
             bool bnd = false;
-            if (act) bnd = hop<PL>(p, s, nl, pl_lane, pl_stride);
+            if (act) bnd = hop<PL>(p, s, nl);
             if (act && bnd) {
                 pending = true;
                 pend_layer = nl;
@@ -718,7 +718,7 @@ extern "C" __global__ void __launch_bounds__(256) mcml_refshape_pass(RefShapeArg
             uint32_t mask = 0;
             int st = ST_ALIVE;
             /* the persistent kernel's step, flat conditions included */
-            const bool bnd = hop<false>(p, s, nl, nullptr, 0);
+            const bool bnd = hop<false>(p, s, nl);
             if (bnd) {
                 c_bnd++;
                 st = cross<false>(p, nl, S, 0, mask, nullptr, 0);
@@ -923,7 +923,7 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
             float s = 0.0f;
             int nl = 0;
             bool bnd = false;
-            if (act) bnd = hop<true>(p, s, nl, pl_lane, REPLAY_BLOCK);
+            if (act) bnd = hop<true>(p, s, nl);
             if (act && bnd) {
                 pending = true;
                 pend_layer = nl;
