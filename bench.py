@@ -9,8 +9,11 @@ and the pathlength dump; inputs are resident and nothing is written to disk.
 
 value = photons / second of wall time over the timed steps (whole job, one GPU).
 roofline = the first-pass transport kernel, FP32 VALU bound: algorithmic FLOP per photon
-(62 per interaction step + 44 per boundary step, counted on the same run) divided by the
-kernel's average duration from HIP events on the library's stream.
+(62 per interaction step + 44 per boundary step, with the per-photon step counts the oracle
+counts at seed 1 over 1e5 photons, committed in tests/golden/algorithmic_counts.json; the
+run's own counters for a configuration not listed there, and as `flop_per_photon_live`)
+times the photons, divided by the kernel's average duration from HIP events on the
+library's stream.
 cpu_baseline = the CPU restatement of the same loop (oracle, OpenMP) on a bounded sample.
 reference_shape = one run of the same photon step in the reference's launch structure
 (32-thread workgroups, one photon per thread, 50 000-step relaunches with per-launch host
@@ -35,6 +38,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 FLOP_INTERACTION = 62  # step 2, boundary test 2, move 6, drop 3, HG 11, sin 3, psi 3, sqrt 3, rotate 20, renorm 9
 FLOP_BOUNDARY = 44     # step, clip, move, Fresnel + Snell on transmission
+# per-photon interaction and boundary counts of each configuration, counted by the oracle at
+# seed 1 over photons [0, 1e5) (tests/golden/make_algorithmic_counts.py)
+COUNTS_FILE = os.path.join(ROOT, "tests", "golden", "algorithmic_counts.json")
 PEAK_FP32_TFLOPS = 157.3
 EXAMPLES = {"head_model": "tests/golden/head_model", "hw8a": "tests/golden/HW8_A"}
 # HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each scaled by what the
@@ -130,7 +136,14 @@ def main():
     n = sim.number_photons
     value = n * args.steps / elapsed
     k_ms = float(np.mean(kern_ms))
-    flop_per_photon = (FLOP_INTERACTION * res.steps_interaction + FLOP_BOUNDARY * res.steps_boundary) / n
+    flop_live = (FLOP_INTERACTION * res.steps_interaction + FLOP_BOUNDARY * res.steps_boundary) / n
+    flop_per_photon, flop_source = flop_live, "this run's step counters"
+    with open(COUNTS_FILE) as f:
+        counts = json.load(f)
+    if args.config in counts["configs"]:
+        c = counts["configs"][args.config]
+        flop_per_photon = (FLOP_INTERACTION * c["interaction"] + FLOP_BOUNDARY * c["boundary"]) / counts["photons"]
+        flop_source = "oracle, seed %d, %d photons" % (counts["seed"], counts["photons"])
     achieved = flop_per_photon * n / (k_ms * 1e-3) / 1e12
     detected = int(res.detected.sum())
     dump_bytes = 2 * detected * (sim.number_layers + 2) * 4  # SoA read + record write
@@ -175,6 +188,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "flop_per_photon": flop_per_photon,
+            "flop_source": flop_source,
+            "flop_per_photon_live": flop_live,
             "traffic": TRANSPORT_TRAFFIC_PER_PHOTON * n,
         },
         "dump": {"bytes": dump_bytes,
@@ -261,7 +276,9 @@ def bench_batch(args):
         "detected": int(res.detected.sum()), "steps_per_photon": res.steps / total,
         "roofline": {"bound": "valu", "kernel": "mcml_first_pass_pl" if args.pathlength else "mcml_first_pass",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRANSPORT_TRAFFIC_PER_PHOTON * total},
+                     "frac": achieved / PEAK_FP32_TFLOPS, "flop_per_photon": flop / total,
+                     "flop_source": "this run's step counters (512 sets)",
+                     "traffic": TRANSPORT_TRAFFIC_PER_PHOTON * total},
     }
     if args.pathlength:
         n_rec = int(res.detected.sum())
