@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--bnd", type=int, default=0)
     ap.add_argument("--pathlength", action="store_true",
                     help="cfg3: also record per-layer pathlengths and dump the records (-R -P)")
+    ap.add_argument("--absorbance", action="store_true",
+                    help="replay into the A_rz/A0_z grids (-R -A -B) on any single-set config (cfg2 always does)")
     ap.add_argument("--train-sets", type=int, default=16384, help="cfg4: simulated training sets")
     ap.add_argument("--train-photons", type=float, default=6e4, help="cfg4: photons per training set")
     ap.add_argument("--white", type=int, default=13, help="cfg4: absorption vectors per set (White MC)")
@@ -114,7 +116,7 @@ def main():
     sim.number_photons = int(args.photons)
     # cfg2 is the absorbance-map workload (-R -A -B: the first pass, then a replay of every
     # detected photon into the A_rz/A0_z grids); the others dump pathlengths (-R -P -B)
-    absorb = args.config == "cfg2"
+    absorb = args.config == "cfg2" or args.absorbance
     opts = M.make_opts(replay=True, pathlength=not absorb, absorbance=absorb, binary=True, block_size=args.block,
                        chunk=args.chunk, refill_min=args.refill, blocks_per_cu=args.bpc,
                        boundary_min=args.bnd)
@@ -164,7 +166,7 @@ def main():
         "config": {
             "workload": "%s: %d-layer slab, %d SDS (%s detection), %.0e photons, %s" % (
                 args.config, sim.number_layers, sim.num_sds, "fiber" if sim.detector_probe_oblique else "ring", n,
-                "replay into %d x %d absorbance grids (%s)" % (sim.grid_nz, sim.grid_nr, flags) if absorb
+                "replay into %d x %d absorbance grids (%s)" % (res.A0_z.shape[-1], res.A_rz.shape[-1], flags) if absorb
                 else "replay + pathlength dump (%s)" % flags),
             "photons": n,
             "layers": sim.number_layers,

@@ -1015,16 +1015,36 @@ extern "C" __global__ void __launch_bounds__(256) mcml_group_offsets(const uint6
 }
 
 /* Sum of detected weight per (set, SDS) group, in FP64 with a fixed reduction order (one
- * workgroup per group), so the result does not depend on scheduling. */
-#define GROUP_BLOCK 256
+ * workgroup per group), so the result depends neither on scheduling nor on the other groups
+ * of the batch.  Each thread keeps GROUP_ILP independent loads and sums in flight, so that a
+ * group of millions of records (one workgroup reading it alone) is not bound by the latency
+ * of one load at a time. */
+#define GROUP_BLOCK 1024
+#define GROUP_ILP 8
 extern "C" __global__ void __launch_bounds__(GROUP_BLOCK) mcml_group_sums(const float* w, const uint64_t* offsets,
                                                                           double* sums)
 {
     __shared__ double part[GROUP_BLOCK];
     const int g = blockIdx.x;
     const uint64_t b = offsets[g], e = offsets[g + 1];
+    constexpr uint64_t T = GROUP_BLOCK;
+    double a[GROUP_ILP];
+#pragma unroll
+    for (int j = 0; j < GROUP_ILP; j++) a[j] = 0.0;
+    uint64_t i = b + threadIdx.x;
+    for (; i + (GROUP_ILP - 1) * T < e; i += GROUP_ILP * T) {
+        float v[GROUP_ILP];
+#pragma unroll
+        for (int j = 0; j < GROUP_ILP; j++) v[j] = w[i + j * T];
+#pragma unroll
+        for (int j = 0; j < GROUP_ILP; j++) a[j] += (double)v[j];
+    }
+#pragma unroll
+    for (int j = 0; j < GROUP_ILP - 1; j++) /* fewer than GROUP_ILP left */
+        if (i + j * T < e) a[j] += (double)w[i + j * T];
     double acc = 0.0;
-    for (uint64_t i = b + threadIdx.x; i < e; i += GROUP_BLOCK) acc += (double)w[i];
+#pragma unroll
+    for (int j = 0; j < GROUP_ILP; j++) acc += a[j];
     part[threadIdx.x] = acc;
     __syncthreads();
     for (int off = GROUP_BLOCK / 2; off > 0; off >>= 1) {
