@@ -325,12 +325,31 @@ typedef struct {
     uint64_t steps, boundary, interaction;
 } oracle_counts;
 
+/* Optional record of every interaction of one photon (test hook): per interaction
+ * ORACLE_TRACE_COLS floats (x, y, z, drop, first, layer), taken where the reference books
+ * the drop (transport.cu:573-600: after the hop, before the spin). */
+#define ORACLE_TRACE_COLS 6
+typedef struct {
+    float* buf;
+    int64_t cap; /* rows */
+    int64_t n;   /* rows seen (may exceed cap) */
+} oracle_trace_buf;
+
 /* One photon through MCd's loop body (transport.cu:370-453) when target == 0, or through
  * MCd_replay's (transport.cu:506-610) when target is an SDS number.  Returns the detection
  * mask (0 = not detected); wout[i] holds the detected weight per SDS.  In replay mode
  * `pl` (L floats) receives the per-layer pathlength and the grids (if non-NULL) the drops. */
+static uint32_t photon_life(const mcml_derived* d, oracle_photon* p, int target, float* wout, float* pl,
+                            uint64_t* A_rz, uint64_t* A0_z, oracle_counts* cnt, oracle_trace_buf* tr);
+
 uint32_t oracle_photon_life(const mcml_derived* d, oracle_photon* p, int target, float* wout, float* pl,
                             uint64_t* A_rz, uint64_t* A0_z, oracle_counts* cnt)
+{
+    return photon_life(d, p, target, wout, pl, A_rz, A0_z, cnt, NULL);
+}
+
+static uint32_t photon_life(const mcml_derived* d, oracle_photon* p, int target, float* wout, float* pl,
+                            uint64_t* A_rz, uint64_t* A0_z, oracle_counts* cnt, oracle_trace_buf* tr)
 {
     const int replay = target != 0;
     uint32_t detected = 0;
@@ -386,6 +405,14 @@ uint32_t oracle_photon_life(const mcml_derived* d, oracle_photon* p, int target,
             float w_temp = rintf(Lc->mua * Lc->mutr * p->w);
             p->w -= w_temp;
             if (cnt) cnt->interaction++;
+            if (tr) {
+                if (tr->n < tr->cap) {
+                    float* row = tr->buf + tr->n * ORACLE_TRACE_COLS;
+                    row[0] = p->x; row[1] = p->y; row[2] = p->z;
+                    row[3] = w_temp; row[4] = (float)p->first_scatter; row[5] = (float)p->layer;
+                }
+                tr->n++;
+            }
             if (replay && A0_z && A_rz) {
                 int zb = zbin_of(d, p->z);
                 if (p->first_scatter) {
@@ -499,6 +526,32 @@ int oracle_replay(const mcml_derived* d, uint64_t seed, uint32_t set, uint64_t k
 }
 
 /* ------------------------------------------------------------------ test hooks */
+/* Replay of photon k for detector `sds` (as oracle_replay, grids optional) that also
+ * records each interaction into trace[cap][ORACLE_TRACE_COLS]; returns the number of
+ * interactions (only the first cap are written), and *hit = 1 when detected again. */
+int64_t oracle_trace(const mcml_derived* d, uint64_t seed, uint32_t set, uint64_t k, int sds, float* trace,
+                     int64_t cap, float* rec, uint64_t* A_rz, uint64_t* A0_z, int* hit)
+{
+    oracle_photon p;
+    float wout[MCML_MAX_SDS + 1];
+    float pl[MCML_MAX_LAYERS];
+    memset(pl, 0, sizeof(pl));
+    oracle_trace_buf tr = {trace, cap, 0};
+    oracle_launch(d, seed, set, k, &p);
+    uint32_t m = photon_life(d, &p, sds, wout, pl, A_rz, A0_z, NULL, &tr);
+    int L = d->num_layers;
+    rec[0] = m ? wout[sds] : 0.0f;
+    for (int l = 0; l < L; l++) rec[1 + l] = pl[l];
+    rec[L + 1] = (float)p.steps;
+    *hit = m ? 1 : 0;
+    return tr.n;
+}
+/* The absorbance cell of a drop at (x, y, z). */
+void oracle_bin(const mcml_derived* d, float x, float y, float z, int* zb, int* rb)
+{
+    *zb = zbin_of(d, z);
+    *rb = rbin_of(d, x, y);
+}
 size_t oracle_photon_size(void) { return sizeof(oracle_photon); }
 void oracle_uniforms(uint64_t seed, uint32_t set, uint64_t k, float* out, int n)
 {

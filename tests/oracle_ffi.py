@@ -63,6 +63,10 @@ def lib():
     L.oracle_detect_state.restype = C.c_uint32
     L.oracle_ring_weight.argtypes = [C.c_float] * 4
     L.oracle_ring_weight.restype = C.c_float
+    L.oracle_trace.argtypes = [D, C.c_uint64, C.c_uint32, C.c_uint64, C.c_int, _fp, C.c_int64, _fp,
+                               C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
+    L.oracle_trace.restype = C.c_int64
+    L.oracle_bin.argtypes = [D, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -98,3 +102,32 @@ def replay(d, seed, k, sds, set_index=0, grids=None):
     a0 = grids[1].ctypes.data if grids is not None else None
     hit = lib().oracle_replay(C.byref(d), seed, set_index, k, sds, rec, a_rz, a0)
     return hit, rec
+
+
+TRACE_COLS = 6  # x, y, z, drop, first, layer
+
+
+def trace(d, seed, k, sds, set_index=0, grids=None):
+    """Replay photon k for detector sds and return (hit, record, interactions), where
+    interactions is an [n, 6] array of (x, y, z, drop, first, layer) per drop."""
+    L = d.num_layers
+    rec = np.zeros(L + 2, dtype=np.float32)
+    cap = 4096
+    while True:
+        buf = np.zeros(cap * TRACE_COLS, dtype=np.float32)
+        a_rz = grids[0].ctypes.data if grids is not None else None
+        a0 = grids[1].ctypes.data if grids is not None else None
+        if grids is not None:
+            grids[0][:] = 0
+            grids[1][:] = 0
+        hit = C.c_int()
+        n = lib().oracle_trace(C.byref(d), seed, set_index, k, sds, buf, cap, rec, a_rz, a0, C.byref(hit))
+        if n <= cap:
+            return hit.value, rec, buf.reshape(cap, TRACE_COLS)[:n]
+        cap = int(n)
+
+
+def bin_of(d, x, y, z):
+    zb, rb = C.c_int(), C.c_int()
+    lib().oracle_bin(C.byref(d), x, y, z, C.byref(zb), C.byref(rb))
+    return zb.value, rb.value
