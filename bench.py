@@ -223,14 +223,17 @@ def main():
 
     if not args.no_cpu:
         import oracle_ffi as O
-        cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
+        # the host share this process may use: its CPU affinity, and OMP_NUM_THREADS where set
+        usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        cores = min(usable, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or usable)
         sample = int(args.cpu_photons) if args.cpu_photons else 150000 * cores
         d = O.derive(sim)
         tc = time.perf_counter()
         O.first_pass(d, args.seed, 0, sample, nthreads=cores)
         dt = time.perf_counter() - tc
         line["cpu_baseline"] = {
-            "value": sample / dt, "unit": "photons/s", "cores": cores, "kind": "port",
+            "value": sample / dt, "unit": "photons/s", "cores": cores, "cores_total": os.cpu_count(),
+            "cores_usable": usable, "kind": "port",
             "sample": "%d photons of the same config, first pass (oracle, OpenMP), %.1f s" % (sample, dt),
         }
     print(json.dumps(line), flush=True)
@@ -385,6 +388,7 @@ def bench_surrogate(args):
         dt = time.perf_counter() - tc
         line["cpu_baseline"] = {"value": reps * n_cpu / dt, "unit": "evals/s",
                                 "cores": int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1),
+                                "cores_total": os.cpu_count(),
                                 "kind": "port", "sample": "%d x %d rows, numpy float64 forward" % (reps, n_cpu)}
     print(json.dumps(line), flush=True)
     net.close()

@@ -148,6 +148,11 @@ typedef struct {
                                 only its kernel arguments between calls */
     int32_t skip_records;    /* 1 = do not return the per-photon detection records (sums,
                                 counts and pathlength rows are still returned) */
+    uint64_t record_cap;     /* initial capacity of the device record buffer, in records;
+                                0 = the most records the run's photons and detectors can
+                                produce (while that fits in a quarter of free device memory).
+                                A first pass that fills a smaller buffer pauses, the buffer
+                                grows and the pass resumes; the records do not depend on it */
 } mcml_opts;
 
 /* First-pass detection record: photon `key` = (set << 40) | photon index, detected by
@@ -189,7 +194,8 @@ typedef struct {
     double ms_dump_kernel;
     double ms_first_pass_wall;
     double ms_replay_wall;
-    int32_t first_pass_launches; /* kernel launches the first pass needed */
+    int32_t first_pass_launches; /* kernel launches the first pass needed (more than one only
+                                    when the record buffer filled and the pass resumed) */
     int32_t block_size, grid_size, chunk, refill_min, boundary_min; /* launch shape used */
 } mcml_result;
 

@@ -94,10 +94,10 @@ def load_config(name):
 
 def make_opts(replay=False, absorbance=False, pathlength=False, avg_pathlength=False, binary=False,
               block_size=0, blocks_per_cu=0, chunk=0, refill_min=0, boundary_min=0, use_graph=False,
-              skip_records=False, reference_shape=False):
+              skip_records=False, reference_shape=False, record_cap=0):
     return abi.Opts(int(replay), int(absorbance), int(pathlength), int(avg_pathlength), int(binary),
                     block_size, blocks_per_cu, chunk, refill_min, boundary_min, int(reference_shape),
-                    int(use_graph), int(skip_records))
+                    int(use_graph), int(skip_records), int(record_cap))
 
 
 class _ResultOwner:

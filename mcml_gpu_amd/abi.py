@@ -94,7 +94,7 @@ class Opts(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "replay", "absorbance", "pathlength", "avg_pathlength", "binary",
         "block_size", "blocks_per_cu", "chunk", "refill_min", "boundary_min", "reference_shape",
-        "use_graph", "skip_records")]
+        "use_graph", "skip_records")] + [("record_cap", C.c_uint64)]
 
 
 class Record(C.Structure):

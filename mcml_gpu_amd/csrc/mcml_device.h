@@ -74,9 +74,11 @@ struct FirstPassArgs {
     float* rec_row;                /* [cap * row_stride]: weight, or [weight, PL_1..PL_L, steps] */
     int32_t row_stride;            /* 1, or L + 2 when the kernel records pathlengths */
     int32_t pad;
-    unsigned long long* n_records; /* records appended (may exceed cap) */
+    unsigned long long* n_records; /* records appended (never beyond cap: see stop_at) */
     uint64_t cap;
     unsigned long long* counters;  /* [CNT_N] */
+    uint64_t stop_at;              /* claim no new tickets once n_records reaches this (~0 = never);
+                                      cap - stop_at covers every photon that can still be run */
 };
 
 struct ReplayArgs {

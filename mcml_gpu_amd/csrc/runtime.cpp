@@ -119,6 +119,29 @@ int ensure(mcml_ctx* c, DevBuf& b, size_t bytes)
     return MCML_OK;
 }
 
+/* Grow b to `bytes`, keeping its first `keep` bytes (a device-to-device copy). */
+int ensure_keep(mcml_ctx* c, DevBuf& b, size_t bytes, size_t keep)
+{
+    if (b.bytes >= bytes) return MCML_OK;
+    if (keep == 0 || b.p == nullptr) return ensure(c, b, bytes);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        set_err(c, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return MCML_ERR_NOMEM;
+    }
+    if ((e = hipMemcpyAsync(p, b.p, keep, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(c->stream)) != hipSuccess) {
+        (void)hipFree(p);
+        set_err(c, "record buffer copy failed: %s", hipGetErrorString(e));
+        return MCML_ERR_DEVICE;
+    }
+    (void)hipFree(b.p);
+    b.p = p;
+    b.bytes = bytes;
+    return MCML_OK;
+}
+
 void release(DevBuf& b)
 {
     if (b.p) (void)hipFree(b.p);
@@ -291,6 +314,42 @@ void fill_devset(const mcml_derived& d, DevSet& S)
     }
 }
 
+
+/* The most detectors of one set that can record the same photon: photons are killed at
+ * their first detection (transport.cu:405-409), so a photon yields at most this many
+ * records.  A fibre accepts only exit points with |x - pos| <= radius / |cos(angle)|, a
+ * ring only distances within [pos - radius, pos + radius]; the answer is the largest
+ * number of these intervals sharing a point (1 when the footprints are disjoint). */
+int max_overlap(const mcml_derived& d)
+{
+    const int S = d.num_sds;
+    std::vector<std::pair<double, int>> ev; /* (coordinate, +1 start / -1 end) */
+    for (int i = 1; i <= S; i++) {
+        double lo, hi;
+        if (d.detector_oblique) {
+            const double ca = fabs((double)d.det_cos_a[i]);
+            if (ca < 0.01) return S;
+            const double half = 1.01 * d.det_radius[i] / ca + 1e-6;
+            lo = d.det_pos[i] - half;
+            hi = d.det_pos[i] + half;
+        } else {
+            lo = (double)d.det_pos[i] - d.det_radius[i] - 1e-6;
+            hi = (double)d.det_pos[i] + d.det_radius[i] + 1e-6;
+        }
+        if (!(lo <= hi)) return S; /* NaN or an inverted interval: no bound but S */
+        ev.push_back({lo, +1});
+        ev.push_back({hi, -1});
+    }
+    std::sort(ev.begin(), ev.end(), [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+        return a.first < b.first || (a.first == b.first && a.second > b.second); /* starts first */
+    });
+    int cur = 0, best = 1;
+    for (const auto& e : ev) {
+        cur += e.second;
+        best = std::max(best, cur);
+    }
+    return std::min(best, S);
+}
 
 } // namespace
 
@@ -486,7 +545,6 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
                mcml_result* out, uint64_t& n_rec)
 {
     const int nsets = (int)der.size();
-    const int S = der[0].num_sds;
     std::vector<DevSet> hs((size_t)nsets);
     std::vector<uint64_t> begin((size_t)nsets + 1, 0);
     for (int j = 0; j < nsets; j++) {
@@ -533,26 +591,55 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
     a.n_records = c->ctrl + 1;
     a.counters = c->ctrl + 2;
 
-    /* record capacity: a quarter of the photons (or every photon times every detector when
-     * that is smaller); on overflow the pass reruns with the exact count -- the result does
-     * not depend on the capacity, each photon's stream is fixed */
-    uint64_t cap = std::min<uint64_t>(total * (uint64_t)S, std::max<uint64_t>(1ull << 20, total / 4 + 1024));
+    /* Record capacity.  Each photon yields at most max_overlap records, so `bound` records
+     * can never overflow; it is used whenever it fits in a quarter of the free device memory
+     * (or the caller's record_cap is smaller).  A smaller buffer makes the kernel pause:
+     * once `stop_at` records exist it claims no new tickets, and cap - stop_at covers every
+     * photon that can still run then -- one per lane, the private ranges of the wavefronts
+     * and one more chunk per wavefront claimed as the limit was crossed.  The buffer then
+     * grows and the pass resumes from the ticket counter.  Records never depend on it:
+     * each photon's stream is fixed. */
+    uint64_t bound = 0;
+    int overlap = 1;
+    for (const mcml_derived& d : der) {
+        const int ov = max_overlap(d);
+        overlap = std::max(overlap, ov);
+        bound += d.number_photons * (uint64_t)ov;
+    }
+    const uint64_t lanes = (uint64_t)grid * (uint64_t)block, waves = lanes / 64;
+    const uint64_t reserve = (lanes + 2 * waves * (uint64_t)a.chunk) * (uint64_t)overlap;
+    const uint64_t per_rec = sizeof(uint64_t) + sizeof(float) * (uint64_t)row_stride;
+    uint64_t cap;
+    if (o.record_cap > 0) {
+        cap = o.record_cap;
+    } else {
+        size_t mem_free = 0, mem_total = 0;
+        HIP_TRY(c, hipMemGetInfo(&mem_free, &mem_total));
+        const uint64_t held = std::min(c->rec_key.bytes / sizeof(uint64_t),
+                                       c->rec_row.bytes / (sizeof(float) * (size_t)row_stride));
+        cap = std::max<uint64_t>({(uint64_t)(mem_free / 4) / per_rec, held, 1ull << 20});
+    }
+    cap = std::max<uint64_t>(cap, reserve + 4096);
+    if (cap >= bound) cap = std::max<uint64_t>(bound, 1);
     unsigned long long cnt[CNT_N] = {0, 0, 0, 0};
     double ms_kernel = 0;
     int launches = 0;
+    n_rec = 0;
     for (;;) {
-        if ((rc = ensure(c, c->rec_key, sizeof(uint64_t) * (size_t)cap)) ||
-            (rc = ensure(c, c->rec_row, sizeof(float) * (size_t)cap * (size_t)row_stride)))
+        if ((rc = ensure_keep(c, c->rec_key, sizeof(uint64_t) * (size_t)cap, sizeof(uint64_t) * (size_t)n_rec)) ||
+            (rc = ensure_keep(c, c->rec_row, sizeof(float) * (size_t)cap * (size_t)row_stride,
+                              sizeof(float) * (size_t)n_rec * (size_t)row_stride)))
             return rc;
         a.rec_key = as<uint64_t>(c->rec_key);
         a.rec_row = as<float>(c->rec_row);
         a.cap = cap;
+        a.stop_at = cap >= bound ? ~0ull : cap - reserve;
         HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-        if (o.use_graph) {
+        if (launches == 0 && o.use_graph) {
             HIP_TRY(c, mcml_graph_first_pass(c->graph, a, grid, block, c->ctrl, kCtrlFirstPass, c->h_ctrl, c->stream));
             HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
         } else {
-            HIP_TRY(c, mcml_launch_reset(c->ctrl, kCtrlFirstPass, c->stream));
+            if (launches == 0) HIP_TRY(c, mcml_launch_reset(c->ctrl, kCtrlFirstPass, c->stream));
             HIP_TRY(c, mcml_launch_first_pass(a, grid, block, c->stream));
             HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
             HIP_TRY(c, hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(unsigned long long) * kCtrlFirstPass,
@@ -561,10 +648,24 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
         launches++;
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         ms_kernel += ms_between(c->ev0, c->ev1);
+        const uint64_t claimed = c->h_ctrl[0];
         n_rec = c->h_ctrl[1];
         for (int i = 0; i < CNT_N; i++) cnt[i] = c->h_ctrl[2 + i];
-        if (n_rec <= cap) break;
-        cap = n_rec + n_rec / 8 + 1024;
+        if (n_rec > cap) {
+            set_err(c, "first pass appended %llu records into a buffer of %llu", (unsigned long long)n_rec,
+                    (unsigned long long)cap);
+            return MCML_ERR_DEVICE;
+        }
+        if (claimed >= total) break;
+        if (claimed == 0 || launches > 64) {
+            set_err(c, "first pass made no progress (%llu of %llu tickets after %d launches)",
+                    (unsigned long long)claimed, (unsigned long long)total, launches);
+            return MCML_ERR_DEVICE;
+        }
+        /* paused: grow toward the records the remaining tickets are expected to add */
+        const double est = (double)n_rec * (double)total / (double)claimed;
+        uint64_t next = std::max<uint64_t>(2 * cap, (uint64_t)(1.25 * est)) + reserve;
+        cap = next >= bound ? std::max<uint64_t>(bound, 1) : next;
     }
     if (n_rec >= (1ull << 32)) {
         set_err(c, "more than 2^32 detection records");

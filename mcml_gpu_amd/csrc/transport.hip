@@ -451,14 +451,21 @@ __device__ __forceinline__ uint32_t set_of_ticket(const uint64_t* set_begin, int
 
 /* Wavefront work queue: lanes named in `need` take consecutive ids from the wavefront's
  * private range [q_next, q_end), which is refilled `chunk` ids at a time from the global
- * counter.  Returns this lane's id, or ~0 when it gets none. */
+ * counter.  Returns this lane's id, or ~0 when it gets none.
+ *
+ * Pause: once `*n_rec` (records appended so far) reaches `stop_at`, the wavefront claims no
+ * further chunk and runs dry as if the counter were exhausted; it still finishes the
+ * photons it holds and the rest of its private range.  The host then grows the record
+ * buffer and launches again; the counter carries on where the paused launch left it, so
+ * every ticket is run exactly once.  stop_at = ~0 disables the check. */
 struct WaveQueue {
     uint64_t next, end;
     bool dry;
 };
 
 __device__ __forceinline__ uint64_t take_ids(WaveQueue& q, uint64_t need, bool mine, unsigned long long* counter,
-                                             uint64_t chunk, uint64_t total, uint64_t& claimed)
+                                             uint64_t chunk, uint64_t total, uint64_t& claimed,
+                                             const unsigned long long* n_rec, uint64_t stop_at)
 {
     claimed = ~0ull;
     const int n = __popcll(need);
@@ -467,9 +474,18 @@ __device__ __forceinline__ uint64_t take_ids(WaveQueue& q, uint64_t need, bool m
     uint64_t id = ~0ull;
     if (mine && rank < avail) id = q.next + rank;
     if ((uint64_t)n > avail) {
-        unsigned long long base = 0;
-        if (lane_id() == (uint32_t)(__ffsll((long long)__ballot(1)) - 1)) base = atomicAdd(counter, chunk);
-        base = bcast_first(base);
+        const uint32_t first = (uint32_t)(__ffsll((long long)__ballot(1)) - 1);
+        bool stop = false;
+        if (stop_at != ~0ull) { /* wave-uniform */
+            unsigned long long r = 0;
+            if (lane_id() == first) r = __hip_atomic_load(n_rec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            stop = bcast_first(r) >= stop_at;
+        }
+        unsigned long long base = total;
+        if (!stop) {
+            if (lane_id() == first) base = atomicAdd(counter, chunk);
+            base = bcast_first(base);
+        }
         uint64_t cend = base + chunk;
         if (cend > total) cend = total;
         if (base >= total) {
@@ -557,7 +573,8 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
         const int ndead = __popcll(dead);
         if (!q.dry && (ndead >= refill_min || ndead == 64)) {
             uint64_t claimed;
-            const uint64_t t = take_ids(q, dead, !alive, a.queue, (uint64_t)a.chunk, a.total, claimed);
+            const uint64_t t = take_ids(q, dead, !alive, a.queue, (uint64_t)a.chunk, a.total, claimed, a.n_records,
+                                        a.stop_at);
             if (claimed != ~0ull && a.nsets > 1) { /* wave-uniform: cache the new chunk's set */
                 cset = set_of_ticket(a.set_begin, a.nsets, claimed);
                 clo = a.set_begin[cset];

@@ -132,3 +132,86 @@ def test_detection_heavy_full_size(ctx):
     # the full run detects about FULL / K times as many photons as the range does
     ratio = len(res.records) / len(rec) / (FULL / K)
     assert 0.9 < ratio < 1.1
+
+
+@pytest.mark.gpu
+def test_readme_example_full_size_photon_by_photon(ctx):
+    """The README example (3 layers, fibres at 0.8/1.5/3.0 cm) at its full 1e6 photons and
+    seed 1: the records and step counts equal the oracle's over all 1e6 photons."""
+    sim = F.config("cfg0")
+    assert sim.number_photons == 1_000_000
+    print("full size: README example 1e6 photons", flush=True)
+    res = ctx.run(sim, make_opts(replay=True, pathlength=True), seed=SEED)
+    d = O.derive(sim)
+    rec, cnt = O.first_pass(d, SEED, 0, sim.number_photons)
+    assert res.photons == sim.number_photons
+    assert (res.steps, res.steps_boundary, res.steps_interaction) == tuple(int(c) for c in cnt)
+    got = res.records[np.lexsort((res.records["sds"], res.records["key"]))]
+    assert np.array_equal(got["key"], rec["key"]) and np.array_equal(got["sds"], rec["sds"])
+    assert np.array_equal(_bits(got["weight"]), _bits(rec["weight"]))
+    _confirm_each_record(res, sim, d)
+
+
+def _group_bounds(records, nsets, S, k_limit):
+    """Sorted records: start of every (set, SDS) group and the end of its photons < k_limit."""
+    key = records["key"]
+    order = ((key >> np.uint64(40)) << np.uint64(45)) | (records["sds"].astype(np.uint64) << np.uint64(40)) | \
+        (key & np.uint64((1 << 40) - 1))
+    assert np.all(order[1:] > order[:-1])  # (set, SDS, photon) order, no duplicates
+    g = np.arange(nsets * S, dtype=np.uint64)
+    base = ((g // np.uint64(S)) << np.uint64(45)) | ((g % np.uint64(S) + np.uint64(1)) << np.uint64(40))
+    return np.searchsorted(order, base), np.searchsorted(order, base + np.uint64(k_limit))
+
+
+@pytest.mark.gpu
+def test_batch_512_sets_full_size(ctx):
+    """The 512-set batch at its real size (budgets log-uniform 1e5-1e7, about 9.8e8 photons)
+    in one persistent launch: every ticket runs once, per-(set, SDS) counts and sums follow
+    from the records, the records of photons [0, 1000) of every set equal the oracle's first
+    pass with that set index, and the autotuned launch shape through the HIP graph returns
+    identical records and step totals."""
+    from mcml_gpu_amd import batch
+    sims, _, budgets = batch.cfg3_sets(512, seed=1)
+    S = sims[0].num_sds
+    print("full size: 512-set batch, %d photons" % int(budgets.sum()), flush=True)
+    res = ctx.run_batch(sims, make_opts(), seed=SEED)
+    assert res.photons == int(budgets.sum()) and res.first_pass_launches == 1
+    recs = res.records
+    lo, hi = _group_bounds(recs, 512, S, 1000)
+    ends = np.append(lo[1:], len(recs))
+    assert np.array_equal((ends - lo).reshape(512, S), res.detected)
+    nonempty = ends > lo  # groups are contiguous: a non-empty group ends where the next one starts
+    sums = np.zeros(len(lo))
+    sums[nonempty] = np.add.reduceat(recs["weight"].astype(np.float64), lo[nonempty])
+    assert np.allclose(sums.reshape(512, S), res.reflectance, rtol=1e-9, atol=0)
+    K = 1000
+    for j, sim in enumerate(sims):
+        rec, _ = O.first_pass(O.derive(sim), SEED, 0, K, set_index=j)
+        mine = np.concatenate([recs[lo[j * S + s]:hi[j * S + s]] for s in range(S)])
+        mine = mine[np.lexsort((mine["sds"], mine["key"]))]
+        assert len(mine) == len(rec), j
+        assert np.array_equal(mine["key"], rec["key"]) and np.array_equal(mine["sds"], rec["sds"]), j
+        assert np.array_equal(_bits(mine["weight"]), _bits(rec["weight"])), j
+    print("full size: 512-set batch again, autotuned shape through the graph", flush=True)
+    o = ctx.autotune(sims, trial_photons=2e7)
+    res2 = ctx.run_batch(sims, make_opts(block_size=o.block_size, chunk=o.chunk, refill_min=o.refill_min,
+                                         boundary_min=o.boundary_min, use_graph=True), seed=SEED)
+    assert (res2.steps, res2.steps_boundary, res2.steps_interaction) == \
+        (res.steps, res.steps_boundary, res.steps_interaction)
+    assert np.array_equal(res2.records, recs)
+
+
+@pytest.mark.gpu
+def test_accept_all_detector_one_launch_full_size(ctx):
+    """A detector that accepts every escaping photon, at 1e8 photons (about 4e7 records):
+    the record buffer is sized from the most records the photons can produce, so the pass
+    takes one launch; the records of photons [0, 1e5) equal the oracle's."""
+    sim = F.hemisphere(FULL, mua=1.0, mus=100.0, g=0.9, n=1.4)
+    print("full size: accept-all hemisphere 1e8 photons", flush=True)
+    res = ctx.run(sim, make_opts(), seed=SEED)
+    assert res.photons == FULL and res.first_pass_launches == 1
+    assert len(res.records) > FULL // 10
+    K = 100_000
+    rec, _ = O.first_pass(O.derive(sim), SEED, 0, K)
+    got = res.records[res.records["key"] < K]
+    assert np.array_equal(got["key"], rec["key"]) and np.array_equal(_bits(got["weight"]), _bits(rec["weight"]))

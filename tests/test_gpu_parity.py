@@ -268,3 +268,26 @@ def test_batch_rejects_mixed_absorbance_grids_and_accepts_empty_sets(ctx):
     assert res.photons == 200 and res.detected[2].sum() == 0
     res = ctx.run(F.ring_busy(0), make_opts(), seed=SEED)
     assert res.photons == 0 and len(res.records) == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pathlength", [False, True], ids=["weights", "inline_pathlength"])
+def test_record_buffer_pause_and_resume(ctx, pathlength):
+    """A record buffer far smaller than the run's records: the first pass pauses when it
+    fills, the buffer grows and the pass resumes from the ticket counter.  Records, rows and
+    step counts equal the single-launch run's and the oracle's."""
+    sim = F.hemisphere(300000, mua=1.0, mus=100.0, g=0.9, n=1.4)
+    base = dict(replay=pathlength, pathlength=pathlength, block_size=64, blocks_per_cu=1, chunk=64)
+    one = ctx.run(sim, make_opts(**base), seed=SEED)
+    assert one.first_pass_launches == 1
+    small = ctx.run(sim, make_opts(record_cap=60000, **base), seed=SEED)
+    print("records", len(one.records), "launches with a small buffer", small.first_pass_launches, flush=True)
+    assert len(one.records) > 100000 and small.first_pass_launches >= 2
+    assert np.array_equal(small.records, one.records)
+    assert (small.photons, small.steps, small.steps_boundary) == (one.photons, one.steps, one.steps_boundary)
+    if pathlength:
+        assert np.array_equal(_bits(small.pathlength), _bits(one.pathlength))
+    rec, cnt = O.first_pass(O.derive(sim), SEED, 0, sim.number_photons)
+    assert small.steps == int(cnt[0])
+    got = photon_order(small.records)
+    assert np.array_equal(got["key"], rec["key"]) and _same_floats(got["weight"], rec["weight"])

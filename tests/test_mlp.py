@@ -155,10 +155,11 @@ def mlp_cls():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "layered"])
-@pytest.mark.parametrize("widths,B", [(FULL, 1000), (ODD, 70), (FULL, 1), (FULL, 4097), ((20, 256, 256, 9), 300)])
+@pytest.mark.parametrize("widths,B", [(FULL, 1000), (ODD, 70), (FULL, 1), (FULL, 4097), ((20, 256, 256, 9), 300),
+                                     (FULL, 100000)])
 def test_forward_matches_bf16_restatement(mlp_cls, widths, B, fused):
     """Both inference paths: the fused kernel (widths <= 32 -> 256 ... -> <= 16) and the
-    layer-by-layer products."""
+    layer-by-layer products, up to the surrogate benchmark's inference batch of 1e5 rows."""
     print("forward", widths, B, "fused" if fused else "layered", flush=True)
     net = mlp_cls(widths, seed=11)
     assert net.use_fused(fused) == (fused and widths != ODD)
