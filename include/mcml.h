@@ -226,6 +226,28 @@ int mcml_run_batch(mcml_ctx* ctx, const mcml_sim* sims, int nsets, const mcml_op
 
 void mcml_result_free(mcml_result* res);
 
+/* mcml_run_batch's per-(set, SDS) detected weight and detected count (its `reflectance` and
+ * `detected` arrays) computed without allocating: the call makes no hipMalloc, hipFree or
+ * host allocation.  No counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ *   d_reflectance [nsets*S] doubles and d_detected [nsets*S] uint64 are DEVICE pointers on
+ *   the context's GPU, written at [set * S + sds - 1] like the result arrays.
+ *   The device work runs in one fixed scratch array per GPU (set table, counters, per-group
+ *   accumulators; MCML_NOALLOC_SCRATCH_MIB MiB, held until the process ends): at most
+ *   MCML_NOALLOC_MAX_SETS sets and MCML_NOALLOC_MAX_GROUPS (set, SDS) groups per call, else
+ *   MCML_ERR_LIMIT.  Only the launch knobs of opts apply; replay, absorbance, pathlength,
+ *   average pathlength, reference_shape and use_graph are rejected with MCML_ERR_ARG.
+ *   Detected weights are summed exactly (128-bit fixed point) and rounded to double once, so
+ *   the sums match mcml_run_batch's FP64 sums to their rounding and do not depend on the
+ *   launch shape; the counts are equal.
+ *   Every call on one GPU uses that GPU's scratch: calls that overlap on one GPU (from two
+ *   threads or two contexts) race.  The call is synchronous like the others. */
+#define MCML_NOALLOC_MAX_SETS 1024
+#define MCML_NOALLOC_MAX_GROUPS 4096
+#define MCML_NOALLOC_SCRATCH_MIB 7
+int mcml_run_batch_sums_noalloc(mcml_ctx* ctx, const mcml_sim* sims, int nsets, const mcml_opts* opts,
+                                uint64_t seed, double* d_reflectance, uint64_t* d_detected);
+
 /* Pick the launch shape (block_size, chunk, refill_min, boundary_min) for these parameter
  * sets: times the first pass of a reduced copy of the batch (about `trial_photons` photons in
  * total) for each candidate shape and writes the fastest into *opts (other fields

@@ -60,6 +60,9 @@ def load_library():
     L.mcml_autotune.argtypes = [C.c_void_p, P(abi.Sim), C.c_int, C.c_uint64, P(abi.Opts)]
     L.mcml_white_rescale.argtypes = [C.c_void_p, P(abi.Sim), P(abi.Result), P(C.c_float), C.c_int,
                                      P(C.c_double), P(C.c_double)]
+    # device pointers for the two outputs
+    L.mcml_run_batch_sums_noalloc.argtypes = [C.c_void_p, P(abi.Sim), C.c_int, P(abi.Opts), C.c_uint64,
+                                              C.c_void_p, C.c_void_p]
     _lib = L
     return L
 

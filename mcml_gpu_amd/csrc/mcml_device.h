@@ -79,7 +79,19 @@ struct FirstPassArgs {
     unsigned long long* counters;  /* [CNT_N] */
     uint64_t stop_at;              /* claim no new tickets once n_records reaches this (~0 = never);
                                       cap - stop_at covers every photon that can still be run */
+    unsigned long long* accum;     /* sums sink (mcml_first_pass_sums): MCML_ACC_SLOTS entries of
+                                      MCML_ACC_WORDS words per (set, SDS) group instead of records */
 };
+
+/* Per-(set, SDS) accumulators of the sums sink.  A group's detected weight is summed exactly
+ * as a 128-bit fixed-point integer with 64 fraction bits (words lo, hi), so the total does
+ * not depend on the order of the additions; `count` counts its records and `bad` those
+ * whose weight is NaN (the reference's pos = 0 ring quirk), which make the sum NaN.  Each
+ * group has MCML_ACC_SLOTS copies, one 64-byte line each, picked by workgroup, to spread the
+ * atomics. */
+#define MCML_ACC_SLOTS 16
+#define MCML_ACC_WORDS 8
+enum { ACC_LO = 0, ACC_HI = 1, ACC_COUNT = 2, ACC_BAD = 3 };
 
 struct ReplayArgs {
     const DevSet* sets;

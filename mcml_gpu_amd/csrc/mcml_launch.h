@@ -21,6 +21,20 @@ hipError_t mcml_graph_first_pass(FirstPassGraph& G, const FirstPassArgs& a, int 
                                  unsigned long long* ctrl, int nctrl, void* host_ctrl, hipStream_t st);
 void mcml_graph_release(FirstPassGraph& G);
 hipError_t mcml_launch_first_pass(const FirstPassArgs& a, int grid, int block, hipStream_t st);
+
+/* Allocation-free first pass (sums sink) and its device scratch (limits in mcml.h). */
+#define MCML_NOALLOC_OFF_CTRL 0
+#define MCML_NOALLOC_OFF_SETS 256
+#define MCML_NOALLOC_OFF_BEGIN (MCML_NOALLOC_OFF_SETS + MCML_NOALLOC_MAX_SETS * 2048)
+#define MCML_NOALLOC_OFF_ACC (MCML_NOALLOC_OFF_BEGIN + ((MCML_NOALLOC_MAX_SETS + 1) * 8 + 255) / 256 * 256)
+#define MCML_NOALLOC_SCRATCH_BYTES \
+    (MCML_NOALLOC_OFF_ACC + MCML_NOALLOC_MAX_GROUPS * MCML_ACC_SLOTS * MCML_ACC_WORDS * 8)
+hipError_t mcml_scratch_address(void** p);
+hipError_t mcml_launch_first_pass_sums(const FirstPassArgs& a, int grid, int block, hipStream_t st);
+hipError_t mcml_occupancy_first_pass_sums(const FirstPassArgs& a, int block, int* blocks_per_cu);
+hipError_t mcml_launch_clear(unsigned long long* p, uint64_t n, hipStream_t st);
+hipError_t mcml_launch_sums_finalize(const unsigned long long* accum, int groups, double* reflectance,
+                                     uint64_t* detected, hipStream_t st);
 hipError_t mcml_launch_replay(const ReplayArgs& a, int grid, hipStream_t st);
 hipError_t mcml_launch_dump(const GatherArgs& a, int grid, hipStream_t st);
 hipError_t mcml_launch_groups(const uint64_t* keys, uint64_t n, int nsets, int nsds, uint64_t* offsets,

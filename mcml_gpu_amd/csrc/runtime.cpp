@@ -74,7 +74,14 @@ struct mcml_ctx {
     DevBuf ref_state, ref_active, ref_fibers, ref_spill_key, ref_spill_w; /* reference launch shape */
     void* stage[2] = {nullptr, nullptr}; /* pinned read-back staging, allocated on first use */
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    /* allocation-free entry point: this device's scratch array (its device address, taken
+     * at creation) and a pinned host staging area for its set table and ticket prefix */
+    void* scratch = nullptr;
+    void* na_stage = nullptr;
 };
+
+static_assert(sizeof(DevSet) == 2048, "the scratch layout assumes 2048-byte set entries");
+static_assert(MCML_NOALLOC_SCRATCH_BYTES <= ((size_t)MCML_NOALLOC_SCRATCH_MIB << 20), "scratch size as documented");
 
 namespace {
 
@@ -452,7 +459,9 @@ extern "C" mcml_ctx* mcml_create(int device, int* err)
     }
     c->cu_count = prop.multiProcessorCount;
     snprintf(c->name, sizeof(c->name), "%s", prop.name[0] ? prop.name : prop.gcnArchName);
-    if (hipMalloc((void**)&c->ctrl, sizeof(unsigned long long) * kCtrlWords) != hipSuccess ||
+    if (mcml_scratch_address(&c->scratch) != hipSuccess || c->scratch == nullptr ||
+        hipHostMalloc(&c->na_stage, MCML_NOALLOC_OFF_ACC - MCML_NOALLOC_OFF_SETS, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&c->ctrl, sizeof(unsigned long long) * kCtrlWords) != hipSuccess ||
         hipHostMalloc((void**)&c->h_ctrl, sizeof(unsigned long long) * kCtrlWords, hipHostMallocDefault) !=
             hipSuccess ||
         hipMalloc((void**)&c->d_mult, sizeof(uint32_t) * MCML_MWC_TABLE_SIZE) != hipSuccess ||
@@ -482,6 +491,7 @@ extern "C" void mcml_destroy(mcml_ctx* c)
         if (c->stage[i]) (void)hipHostFree(c->stage[i]);
         if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
     }
+    if (c->na_stage) (void)hipHostFree(c->na_stage);
     if (c->d_mult) (void)hipFree(c->d_mult);
     if (c->ctrl) (void)hipFree(c->ctrl);
     if (c->h_ctrl) (void)hipHostFree(c->h_ctrl);
@@ -1113,6 +1123,122 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
 extern "C" int mcml_run(mcml_ctx* c, const mcml_sim* sim, const mcml_opts* opts, uint64_t seed, mcml_result* out)
 {
     return mcml_run_batch(c, sim, 1, opts, seed, out);
+}
+
+/* mcml_run_batch's reflectance and detected arrays with nothing allocated: the set table
+ * goes through the context's pinned staging area into this device's scratch array, the
+ * transport adds every detection to per-group accumulators there (no records, no sort),
+ * and a last kernel writes the caller's device arrays.  The accumulators and the control
+ * block are cleared by a kernel at the start of every call (the scratch keeps the previous
+ * call's contents). */
+extern "C" int mcml_run_batch_sums_noalloc(mcml_ctx* c, const mcml_sim* sims, int nsets, const mcml_opts* opts,
+                                           uint64_t seed, double* d_reflectance, uint64_t* d_detected)
+{
+    if (!c || !sims || !d_reflectance || !d_detected) return MCML_ERR_ARG;
+    mcml_opts o;
+    memset(&o, 0, sizeof(o));
+    if (opts) o = *opts;
+    if (o.replay || o.absorbance || o.pathlength || o.avg_pathlength || o.reference_shape || o.use_graph) {
+        set_err(c, "the allocation-free first pass returns per-(set, SDS) sums only: replay, absorbance, "
+                   "pathlength, reference_shape and use_graph are not available");
+        return MCML_ERR_ARG;
+    }
+    if (nsets < 1) {
+        set_err(c, "nsets must be >= 1");
+        return MCML_ERR_ARG;
+    }
+    if (nsets > MCML_NOALLOC_MAX_SETS) {
+        set_err(c, "%d parameter sets: the allocation-free call takes at most %d", nsets, MCML_NOALLOC_MAX_SETS);
+        return MCML_ERR_LIMIT;
+    }
+    if (!c->scratch || !c->na_stage) {
+        set_err(c, "no scratch array on this device");
+        return MCML_ERR_DEVICE;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    DevSet* hs = static_cast<DevSet*>(c->na_stage);
+    uint64_t* begin = reinterpret_cast<uint64_t*>(static_cast<char*>(c->na_stage) +
+                                                  (MCML_NOALLOC_OFF_BEGIN - MCML_NOALLOC_OFF_SETS));
+    begin[0] = 0;
+    int L = 0, S = 0;
+    for (int j = 0; j < nsets; j++) {
+        mcml_derived d;
+        const int rc = mcml_derive(&sims[j], &d);
+        if (rc != MCML_OK) {
+            set_err(c, "set %d: layers must be 1..%d and SDS 1..%d", j, MCML_MAX_LAYERS, MCML_MAX_SDS);
+            return rc;
+        }
+        if (j == 0) {
+            L = d.num_layers;
+            S = d.num_sds;
+        } else if (d.num_layers != L || d.num_sds != S) {
+            set_err(c, "set %d: all sets of a batch must share number_layers and num_SDS", j);
+            return MCML_ERR_ARG;
+        }
+        if (sims[j].number_photons >= (1ull << 40)) {
+            set_err(c, "set %d: number_photons must be below 2^40", j);
+            return MCML_ERR_LIMIT;
+        }
+        fill_devset(d, hs[j]);
+        hs[j].ticket_begin = begin[j];
+        begin[j + 1] = begin[j] + d.number_photons;
+    }
+    const int groups = nsets * S;
+    if (groups > MCML_NOALLOC_MAX_GROUPS) {
+        set_err(c, "%d (set, SDS) groups: the allocation-free call takes at most %d", groups, MCML_NOALLOC_MAX_GROUPS);
+        return MCML_ERR_LIMIT;
+    }
+    const int block = o.block_size > 0 ? o.block_size : 256;
+    if (block % 64 != 0 || block > 256) {
+        set_err(c, "block_size must be a multiple of 64 and at most 256");
+        return MCML_ERR_ARG;
+    }
+    char* scr = static_cast<char*>(c->scratch);
+    unsigned long long* ctrl = reinterpret_cast<unsigned long long*>(scr + MCML_NOALLOC_OFF_CTRL);
+    DevSet* d_sets = reinterpret_cast<DevSet*>(scr + MCML_NOALLOC_OFF_SETS);
+    uint64_t* d_begin = reinterpret_cast<uint64_t*>(scr + MCML_NOALLOC_OFF_BEGIN);
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(scr + MCML_NOALLOC_OFF_ACC);
+    const uint64_t acc_words = (uint64_t)groups * MCML_ACC_SLOTS * MCML_ACC_WORDS;
+    HIP_TRY(c, hipMemcpyAsync(d_sets, hs, sizeof(DevSet) * (size_t)nsets, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(d_begin, begin, sizeof(uint64_t) * ((size_t)nsets + 1), hipMemcpyHostToDevice,
+                              c->stream));
+    FirstPassArgs a;
+    memset(&a, 0, sizeof(a));
+    a.sets = d_sets;
+    a.set_begin = d_begin;
+    a.nsets = nsets;
+    a.num_layers = L;
+    a.row_stride = 1;
+    a.lds_sets = sizeof(DevSet) * (size_t)nsets <= kLdsTableMax ? 1 : 0;
+    int occ = 0;
+    HIP_TRY(c, mcml_occupancy_first_pass_sums(a, block, &occ));
+    if (occ < 1) occ = 1;
+    const int bpc = o.blocks_per_cu > 0 ? std::min(o.blocks_per_cu, occ) : occ;
+    const int grid = c->cu_count * bpc;
+    a.chunk = o.chunk > 0 ? std::max(o.chunk, 64) : kDefaultChunk;
+    a.refill_min = o.refill_min > 0 ? std::min(o.refill_min, 64) : kDefaultRefill;
+    a.bnd_min = o.boundary_min > 0 ? std::min(o.boundary_min, 64) : kDefaultBoundary;
+    a.total = begin[nsets];
+    a.seed = seed;
+    a.mult = c->d_mult;
+    a.queue = ctrl + 0;
+    a.n_records = ctrl + 1;
+    a.counters = ctrl + 2;
+    a.stop_at = ~0ull;
+    a.accum = acc;
+    HIP_TRY(c, mcml_launch_clear(ctrl, kCtrlWords, c->stream));
+    HIP_TRY(c, mcml_launch_clear(acc, acc_words, c->stream));
+    HIP_TRY(c, mcml_launch_first_pass_sums(a, grid, block, c->stream));
+    HIP_TRY(c, mcml_launch_sums_finalize(acc, groups, d_reflectance, d_detected, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->h_ctrl, ctrl, sizeof(unsigned long long) * kCtrlFirstPass, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->h_ctrl[2 + CNT_PHOTONS] != begin[nsets]) {
+        set_err(c, "allocation-free first pass launched %llu photons, expected %llu",
+                (unsigned long long)c->h_ctrl[2 + CNT_PHOTONS], (unsigned long long)begin[nsets]);
+        return MCML_ERR_DEVICE;
+    }
+    return MCML_OK;
 }
 
 /* calculate_average_pathlength, transport.cu:274-297 */

@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "mcml_detmath.h"
 #include "mcml_device.h"
 #include "mwc_multipliers.h"
@@ -532,10 +534,40 @@ __device__ __forceinline__ uint32_t set_for(const uint64_t* set_begin, int nsets
     return set_of_ticket(set_begin, nsets, t);
 }
 
+/* Add a detected weight to its group's accumulator slot (see MCML_ACC_SLOTS): the float's
+ * exact value as a fixed-point integer with 64 fraction bits, in two words with the carry of
+ * the low word propagated by the caller that produced it, so the total is exact in any order.
+ * Weights are below 2^32 W-units, so 2^32 records cannot overflow 128 bits; values below
+ * 2^-41 W-units lose the bits beyond 2^-64. */
+__device__ __forceinline__ void acc_weight(unsigned long long* e, float w)
+{
+    const uint32_t b = __float_as_uint(w);
+    const uint32_t ex = b >> 23;
+    atomicAdd(&e[ACC_COUNT], 1ull);
+    if ((b >> 31) != 0u || ex == 255u) { /* NaN, infinity or negative */
+        atomicAdd(&e[ACC_BAD], 1ull);
+        return;
+    }
+    if (ex == 0u) return; /* subnormal: below 2^-126, nothing to add at 2^-64 resolution */
+    const uint64_t m = (uint64_t)((b & 0x7FFFFFu) | 0x800000u);
+    const int sh = (int)ex - 86; /* value * 2^64 = m * 2^(ex - 150 + 64) */
+    uint64_t lo = 0, hi = 0;
+    if (sh >= 64) hi = m << (sh - 64);
+    else if (sh > 40) { lo = m << sh; hi = m >> (64 - sh); }
+    else if (sh >= 0) lo = m << sh;
+    else if (sh > -24) lo = m >> (-sh);
+    if (lo != 0) {
+        const unsigned long long old = atomicAdd(&e[ACC_LO], (unsigned long long)lo);
+        if (old + lo < old) hi++; /* the low word wrapped: carry into the high word */
+    }
+    if (hi != 0) atomicAdd(&e[ACC_HI], (unsigned long long)hi);
+}
+
 /* First pass over all photon tickets (MCd + LaunchPhoton_Global + calculate_reflectance,
  * transport.cu:341-471, 722-739, 299-311).  PL = also accumulate per-layer pathlength and
- * step count, so that each detection record is already the replay's output row. */
-template <bool PL>
+ * step count, so that each detection record is already the replay's output row.  SUMS =
+ * no records: each detection is added to its (set, SDS) group's accumulators instead. */
+template <bool PL, bool SUMS>
 __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned char* lds_raw)
 {
     const DevSet* sets = a.sets;
@@ -628,7 +660,16 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
             c_steps += (uint32_t)p.steps;
             c_int += n_int;
             n_int = 0;
-            if (st == ST_DETECTED) {
+            if (SUMS && st == ST_DETECTED) {
+                const DevSet* S = sets + set;
+                const int ns = S->num_sds;
+                unsigned long long* slot = a.accum + (uint64_t)(blockIdx.x & (MCML_ACC_SLOTS - 1)) * MCML_ACC_WORDS;
+                for (int i = 1; i <= ns; i++) {
+                    if (!(mask & (1u << i))) continue;
+                    const uint64_t grp = (uint64_t)set * (uint64_t)ns + (uint64_t)(i - 1);
+                    acc_weight(slot + grp * (MCML_ACC_SLOTS * MCML_ACC_WORDS), credited(p, S, i));
+                }
+            } else if (st == ST_DETECTED) {
                 const unsigned long long base = atomicAdd(a.n_records, (unsigned long long)__popc(mask));
                 unsigned long long idx = base;
                 const DevSet* S = sets + set;
@@ -670,7 +711,15 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass(FirstPassArgs a)
 {
     stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    first_pass_body<false>(a, lds_raw);
+    first_pass_body<false, false>(a, lds_raw);
+}
+
+/* The same pass adding each detection to its group's accumulators (no records). */
+extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_sums(FirstPassArgs a)
+{
+    stage_tables();
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    first_pass_body<false, true>(a, lds_raw);
 }
 
 /* The same pass recording per-layer pathlength (replay without absorbance maps needs no
@@ -679,7 +728,7 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_pl(FirstPassArgs a)
 {
     stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    first_pass_body<true>(a, lds_raw);
+    first_pass_body<true, false>(a, lds_raw);
 }
 
 /* ------------------------------------------------------- reference launch shape */
@@ -1071,6 +1120,33 @@ extern "C" __global__ void __launch_bounds__(GROUP_BLOCK) mcml_group_sums(const 
     if (threadIdx.x == 0) sums[g] = part[0];
 }
 
+/* Per-group totals of the sums sink: the slots' 128-bit sums added exactly, then rounded
+ * to double; NaN when a record's weight was not a finite non-negative number. */
+extern "C" __global__ void __launch_bounds__(256) mcml_sums_finalize(const unsigned long long* accum, int groups,
+                                                                      double* reflectance, uint64_t* detected)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= groups) return;
+    const unsigned long long* e = accum + (uint64_t)g * (MCML_ACC_SLOTS * MCML_ACC_WORDS);
+    uint64_t lo = 0, hi = 0, n = 0, bad = 0;
+    for (int k = 0; k < MCML_ACC_SLOTS; k++, e += MCML_ACC_WORDS) {
+        const uint64_t l = e[ACC_LO];
+        lo += l;
+        hi += e[ACC_HI] + (lo < l ? 1u : 0u);
+        n += e[ACC_COUNT];
+        bad += e[ACC_BAD];
+    }
+    reflectance[g] = bad ? __builtin_nan("") : (double)hi + (double)lo * 0x1p-64;
+    detected[g] = n;
+}
+
+/* Zero n words with a kernel (no memset node, so a caller may capture it in a graph). */
+extern "C" __global__ void __launch_bounds__(256) mcml_clear_words(unsigned long long* p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
 /* ----------------------------------------------------------------- counters reset */
 /* Zero the control block [queue, n_records, counters...] with a kernel rather than a
  * memset, so the same sequence can be recorded in a HIP graph. */
@@ -1153,6 +1229,46 @@ void mcml_graph_release(FirstPassGraph& G)
     if (G.exec) (void)hipGraphExecDestroy(G.exec);
     if (G.graph) (void)hipGraphDestroy(G.graph);
     G = FirstPassGraph();
+}
+
+/* Device scratch of the allocation-free entry point (mcml_run_batch_sums_noalloc): the
+ * control block, the set table, the ticket prefix and the group accumulators.  Each GPU
+ * holds its own copy from the code object's load until the process ends; host code reaches
+ * it only through the address mcml_scratch_address returns for the current device. */
+static __device__ __align__(256) unsigned char scratch[MCML_NOALLOC_SCRATCH_BYTES];
+
+hipError_t mcml_scratch_address(void** p)
+{
+    *p = nullptr;
+    return hipGetSymbolAddress(p, HIP_SYMBOL(scratch));
+}
+
+hipError_t mcml_launch_first_pass_sums(const FirstPassArgs& a, int grid, int block, hipStream_t st)
+{
+    const size_t lds = first_pass_lds(a, block);
+    hipLaunchKernelGGL(mcml_first_pass_sums, dim3(grid), dim3(block), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t mcml_occupancy_first_pass_sums(const FirstPassArgs& a, int block, int* blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_first_pass_sums, block,
+                                                         first_pass_lds(a, block));
+}
+
+hipError_t mcml_launch_clear(unsigned long long* p, uint64_t n, hipStream_t st)
+{
+    const uint64_t grid = std::min<uint64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(mcml_clear_words, dim3((unsigned)std::max<uint64_t>(grid, 1)), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t mcml_launch_sums_finalize(const unsigned long long* accum, int groups, double* reflectance,
+                                     uint64_t* detected, hipStream_t st)
+{
+    hipLaunchKernelGGL(mcml_sums_finalize, dim3((groups + 255) / 256), dim3(256), 0, st, accum, groups, reflectance,
+                       detected);
+    return hipGetLastError();
 }
 
 hipError_t mcml_launch_first_pass(const FirstPassArgs& a, int grid, int block, hipStream_t st)
