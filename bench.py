@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--bpc", type=int, default=0)
     ap.add_argument("--bnd", type=int, default=0)
+    ap.add_argument("--spill-below", type=int, default=0,
+                    help="drain compaction threshold (0 = library default, -1 = off)")
     ap.add_argument("--pathlength", action="store_true",
                     help="cfg3: also record per-layer pathlengths and dump the records (-R -P)")
     ap.add_argument("--absorbance", action="store_true",
@@ -119,7 +121,7 @@ def main():
     absorb = args.config == "cfg2" or args.absorbance
     opts = M.make_opts(replay=True, pathlength=not absorb, absorbance=absorb, binary=True, block_size=args.block,
                        chunk=args.chunk, refill_min=args.refill, blocks_per_cu=args.bpc,
-                       boundary_min=args.bnd)
+                       boundary_min=args.bnd, spill_below=args.spill_below)
     flags = "-R -A -B" if absorb else "-R -P -B"
     ctx = M.Context(0)
     for _ in range(args.warmup):
@@ -196,6 +198,21 @@ def main():
         },
         "dump": {"bytes": dump_bytes,
                  "GB_per_s": (dump_bytes / (np.mean(dump_ms) * 1e-3) / 1e9) if np.mean(dump_ms) > 0 else None},
+    }
+
+    # where the lanes' time goes: one more run of the first pass's statistics variant (not
+    # timed with the steps above): lane-blocks by state and the drain after the queue ran dry
+    rs = ctx.run(sim, M.make_opts(replay=True, pathlength=not absorb, absorbance=absorb, binary=True,
+                                  block_size=args.block, chunk=args.chunk, refill_min=args.refill,
+                                  blocks_per_cu=args.bpc, boundary_min=args.bnd, spill_below=args.spill_below,
+                                  lane_stats=True), seed=args.seed)
+    lb = rs.lane_blocks.astype(np.float64)
+    line["lanes"] = {
+        "lane_blocks": [int(v) for v in rs.lane_blocks],
+        "frac_stepping": lb[0] / lb.sum(), "frac_boundary_wait": lb[1] / lb.sum(),
+        "frac_refill_wait": lb[2] / lb.sum(), "frac_dry": lb[3] / lb.sum(),
+        "steps_per_stepping_lane_block": rs.steps / lb[0] if lb[0] else None,
+        "ms_drain": rs.ms_drain, "ms_clock": rs.ms_clock, "kernel_ms": rs.ms_first_pass_kernel,
     }
 
     if not args.no_refshape:

@@ -153,6 +153,12 @@ typedef struct {
                                 produce (while that fits in a quarter of free device memory).
                                 A first pass that fills a smaller buffer pauses, the buffer
                                 grows and the pass resumes; the records do not depend on it */
+    int32_t lane_stats;      /* 1 = run the first pass's statistics variant and fill
+                                mcml_result.lane_blocks, ms_drain and ms_clock (a few percent
+                                slower; not with use_graph) */
+    int32_t spill_below;     /* drain compaction: once the photon queue is exhausted, a
+                                wavefront left with at most this many photons hands them to a
+                                compacted follow-up launch; 0 = default (32), -1 = off */
 } mcml_opts;
 
 /* First-pass detection record: photon `key` = (set << 40) | photon index, detected by
@@ -189,7 +195,7 @@ typedef struct {
     uint64_t steps_interaction; /* steps with absorption + scattering */
     /* timings, milliseconds: kernels by HIP events on the library stream, walls by
      * steady clock around the whole phase (allocation, copies, host reductions) */
-    double ms_first_pass_kernel;
+    double ms_first_pass_kernel;  /* HIP events around all the first pass's launches */
     double ms_replay_kernel;
     double ms_dump_kernel;
     double ms_first_pass_wall;
@@ -197,6 +203,16 @@ typedef struct {
     int32_t first_pass_launches; /* kernel launches the first pass needed (more than one only
                                     when the record buffer filled and the pass resumed) */
     int32_t block_size, grid_size, chunk, refill_min, boundary_min; /* launch shape used */
+    /* opts.lane_stats only.  lane_blocks: lane-blocks (one lane for one block of photon steps
+     * between two wavefront decisions) by the lane's state when the block began: [0] stepping,
+     * [1] waiting at a boundary, [2] waiting for a refill, [3] idle because the ticket queue
+     * had run dry.  ms_drain: from the first wavefront finding the queue dry to the last
+     * wavefront's exit; ms_clock: from the first wavefront's start to the last one's exit
+     * (the GPU's real-time clock, first launch of the pass). */
+    uint64_t lane_blocks[4];
+    double ms_drain;
+    double ms_clock;
+    int32_t drain_launches;     /* follow-up launches that continued spilled photons */
 } mcml_result;
 
 typedef struct mcml_ctx mcml_ctx;
@@ -209,6 +225,13 @@ const char* mcml_last_error(const mcml_ctx* ctx);
 /* Device name as reported by the runtime (summary.json "sim_GPU", io.cpp:364). */
 const char* mcml_device_name(const mcml_ctx* ctx);
 int mcml_device_count(int* count);
+
+/* Self-test of the device arithmetic the transport relies on (no counterpart in the
+ * reference): the hardware reciprocal and square root with their one-step corrections must
+ * give exactly IEEE 1.0f / x (every significand at 16 exponents) and sqrtf(x) (every float in
+ * [2^-126, 4)), as the CPU restatement computes them.  mismatches[0], [1] receive the counts.
+ * mcml_create runs it and fails with MCML_ERR_DEVICE on any mismatch. */
+int mcml_selftest_math(mcml_ctx* ctx, uint64_t* mismatches);
 
 /* Derive the per-simulation constants (io.cpp:179-237). */
 int mcml_derive(const mcml_sim* sim, mcml_derived* out);

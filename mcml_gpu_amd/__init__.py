@@ -60,6 +60,7 @@ def load_library():
     L.mcml_autotune.argtypes = [C.c_void_p, P(abi.Sim), C.c_int, C.c_uint64, P(abi.Opts)]
     L.mcml_white_rescale.argtypes = [C.c_void_p, P(abi.Sim), P(abi.Result), P(C.c_float), C.c_int,
                                      P(C.c_double), P(C.c_double)]
+    L.mcml_selftest_math.argtypes = [C.c_void_p, P(C.c_uint64)]
     # device pointers for the two outputs
     L.mcml_run_batch_sums_noalloc.argtypes = [C.c_void_p, P(abi.Sim), C.c_int, P(abi.Opts), C.c_uint64,
                                               C.c_void_p, C.c_void_p]
@@ -97,10 +98,10 @@ def load_config(name):
 
 def make_opts(replay=False, absorbance=False, pathlength=False, avg_pathlength=False, binary=False,
               block_size=0, blocks_per_cu=0, chunk=0, refill_min=0, boundary_min=0, use_graph=False,
-              skip_records=False, reference_shape=False, record_cap=0):
+              skip_records=False, reference_shape=False, record_cap=0, lane_stats=False, spill_below=0):
     return abi.Opts(int(replay), int(absorbance), int(pathlength), int(avg_pathlength), int(binary),
                     block_size, blocks_per_cu, chunk, refill_min, boundary_min, int(reference_shape),
-                    int(use_graph), int(skip_records), int(record_cap))
+                    int(use_graph), int(skip_records), int(record_cap), int(lane_stats), int(spill_below))
 
 
 class _ResultOwner:
@@ -162,8 +163,10 @@ class RunResult:
                 self.A_rz = self.A0_z = None
         for f in ("photons", "steps", "steps_boundary", "steps_interaction", "ms_first_pass_kernel",
                   "ms_replay_kernel", "ms_dump_kernel", "ms_first_pass_wall", "ms_replay_wall",
-                  "first_pass_launches", "block_size", "grid_size", "chunk", "refill_min", "boundary_min"):
+                  "first_pass_launches", "block_size", "grid_size", "chunk", "refill_min", "boundary_min",
+                  "ms_drain", "ms_clock", "drain_launches"):
             setattr(self, f, getattr(r, f))
+        self.lane_blocks = np.array(list(r.lane_blocks), dtype=np.uint64)
 
     def pathlength_records(self, set_index, sds):
         """[weight, PL_1..PL_L, steps] rows of one (set, 1-based SDS) in photon order."""

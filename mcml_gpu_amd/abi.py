@@ -94,7 +94,8 @@ class Opts(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "replay", "absorbance", "pathlength", "avg_pathlength", "binary",
         "block_size", "blocks_per_cu", "chunk", "refill_min", "boundary_min", "reference_shape",
-        "use_graph", "skip_records")] + [("record_cap", C.c_uint64)]
+        "use_graph", "skip_records")] + [("record_cap", C.c_uint64), ("lane_stats", C.c_int32),
+                                         ("spill_below", C.c_int32)]
 
 
 class Record(C.Structure):
@@ -133,6 +134,10 @@ class Result(C.Structure):
         ("chunk", C.c_int32),
         ("refill_min", C.c_int32),
         ("boundary_min", C.c_int32),
+        ("lane_blocks", C.c_uint64 * 4),
+        ("ms_drain", C.c_double),
+        ("ms_clock", C.c_double),
+        ("drain_launches", C.c_int32),
     ]
 
 

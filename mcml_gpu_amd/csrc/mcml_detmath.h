@@ -17,8 +17,7 @@
  *   - sin/cos of 2*pi*u for launch angles: exact quarter-turn reduction + near-minimax
  *     polynomials on |a| <= pi/4 (fit_sincos.py; reduction after Cody & Waite, "Software
  *     Manual for the Elementary Functions", 1980);
- *   - acos: asin Taylor series (A&S 4.4.40) on [0, 1/2] and acos(x) = 2 asin(sqrt((1-x)/2));
- *   - 1/x and 1/sqrt(x): bit-pattern seeds and three Newton steps.
+ *   - acos: asin Taylor series (A&S 4.4.40) on [0, 1/2] and acos(x) = 2 asin(sqrt((1-x)/2)).
  * The asin coefficients are the exact rational Taylor coefficients rounded to binary32; the
  * launch sin/cos coefficients come from fit_sincos.py.
  * None of these reproduce any vendor math library; they are accurate to a few ulp and,
@@ -65,34 +64,6 @@ MCML_HD float mcml_one_plus(uint32_t x) { return mcml_u2f(0x3F800000u | (x >> 9)
 MCML_HD float mcml_uniform(mcml_mwc* r)
 {
     return 2.0f - mcml_one_plus(mcml_mwc_next(r));
-}
-
-/* 1/x for positive normal x: a bit-pattern seed (relative error below 5.1 %) refined by
- * three Newton steps e = 1 - x r, r = r + r e, each a correctly rounded fma, so CPU and GPU
- * agree bit for bit.  Within 0.51 ulp of 1/x: seven operations against the ten of a
- * correctly rounded division. */
-MCML_HD float mcml_rcp(float x)
-{
-    float r = mcml_u2f(0x7EF311C3u - mcml_f2u(x));
-    float e = fmaf(-x, r, 1.0f);
-    r = fmaf(r, e, r);
-    e = fmaf(-x, r, 1.0f);
-    r = fmaf(r, e, r);
-    e = fmaf(-x, r, 1.0f);
-    return fmaf(r, e, r);
-}
-
-/* 1/sqrt(x) for x >= 0 (finite, not subnormal): a bit-pattern seed (relative error below
- * 7.2 %) and three Newton steps y += y (1/2 - (x/2) y^2) in fmas; within 1 ulp.  x = 0
- * gives a large finite value, so x * mcml_rsqrt(x) = 0 there.  Twelve full-rate operations
- * against the quarter-rate square root and fix-up a correctly rounded sqrtf compiles to. */
-MCML_HD float mcml_rsqrt(float x)
-{
-    float y = mcml_u2f(0x5F3759DFu - (mcml_f2u(x) >> 1));
-    const float h = 0.5f * x;
-    y = fmaf(y, fmaf(-h * y, y, 0.5f), y);
-    y = fmaf(y, fmaf(-h * y, y, 0.5f), y);
-    return fmaf(y, fmaf(-h * y, y, 0.5f), y);
 }
 
 MCML_HD uint64_t mcml_mix64(uint64_t z)

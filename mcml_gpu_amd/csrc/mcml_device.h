@@ -49,6 +49,13 @@ struct alignas(16) DevSet {
 /* Work counters accumulated by the first-pass kernel (one atomic per lane at exit). */
 enum { CNT_PHOTONS = 0, CNT_STEPS = 1, CNT_BOUNDARY = 2, CNT_INTERACTION = 3, CNT_N = 4 };
 
+/* Lane statistics of the statistics variant of the first pass (mcml_opts.lane_stats):
+ * lane-blocks (one lane for one block of STEPS_PER_CHECK steps) by what the lane was doing
+ * when the block began, and real-time clock stamps (the two minima stored complemented, so
+ * that all three are atomic maxima over a zeroed block). */
+enum { STAT_STEP = 0, STAT_PENDING = 1, STAT_REFILL = 2, STAT_DRY = 3, STAT_T_START = 4, STAT_T_DRY = 5,
+       STAT_T_END = 6, STAT_N = 7 };
+
 /* Detection-record sort key: set (19 bits) | SDS (5 bits) | photon index (40 bits), so
  * that ascending keys group records by (set, SDS) in photon order -- the order of the
  * pathlength_SDS_# files. */
@@ -81,7 +88,28 @@ struct FirstPassArgs {
                                       cap - stop_at covers every photon that can still be run */
     unsigned long long* accum;     /* sums sink (mcml_first_pass_sums): MCML_ACC_SLOTS entries of
                                       MCML_ACC_WORDS words per (set, SDS) group instead of records */
+    unsigned long long* stats;     /* [STAT_N] lane statistics (statistics variant only) */
+    /* Drain compaction.  Once the ticket queue is exhausted, a wavefront with at most
+     * spill_below lanes still holding photons writes those photons to `spill` and exits,
+     * so that sparse wavefronts stop competing for their SIMDs; the host continues the
+     * spilled photons in a launch with resume = 1, where tickets index `resume_buf` instead
+     * of new photons (and which may spill again). */
+    float* spill;                  /* [SPILL_FIELDS(L) x spill_cap] SoA photon states (NULL = no spill) */
+    unsigned long long* n_spill;   /* photons written to `spill` */
+    uint64_t spill_cap;            /* column stride of `spill`: at least every lane of the launch */
+    const float* resume_buf;       /* resume = 1: the photons to continue, SoA as `spill` */
+    uint64_t resume_cap;           /* column stride of resume_buf */
+    int32_t spill_below;
+    int32_t resume;
 };
+
+/* Fields of one spilled photon (each a 32-bit column of the SoA spill arrays): position,
+ * direction, weight, layer, steps, first-scatter flag, the RNG state (3 words), the running
+ * pathlength and its layer, the record key (2 words), the pending boundary, then the
+ * per-layer pathlength totals. */
+enum { SP_X = 0, SP_Y, SP_Z, SP_DX, SP_DY, SP_DZ, SP_W, SP_LAYER, SP_STEPS, SP_FIRST, SP_RNG_LO, SP_RNG_HI, SP_RNG_A,
+       SP_PL_CUR, SP_PL_LAYER, SP_KEY_LO, SP_KEY_HI, SP_PENDING, SP_PL0 };
+#define SPILL_FIELDS(L) (SP_PL0 + (L))
 
 /* Per-(set, SDS) accumulators of the sums sink.  A group's detected weight is summed exactly
  * as a 128-bit fixed-point integer with 64 fraction bits (words lo, hi), so the total does

@@ -30,6 +30,7 @@ hipError_t mcml_launch_first_pass(const FirstPassArgs& a, int grid, int block, h
 #define MCML_NOALLOC_SCRATCH_BYTES \
     (MCML_NOALLOC_OFF_ACC + MCML_NOALLOC_MAX_GROUPS * MCML_ACC_SLOTS * MCML_ACC_WORDS * 8)
 hipError_t mcml_scratch_address(void** p);
+hipError_t mcml_launch_selftest(unsigned long long* bad, hipStream_t st);
 hipError_t mcml_launch_first_pass_sums(const FirstPassArgs& a, int grid, int block, hipStream_t st);
 hipError_t mcml_occupancy_first_pass_sums(const FirstPassArgs& a, int block, int* blocks_per_cu);
 hipError_t mcml_launch_clear(unsigned long long* p, uint64_t n, hipStream_t st);

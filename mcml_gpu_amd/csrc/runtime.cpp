@@ -39,8 +39,18 @@ constexpr int kDefaultChunk = 128;
 constexpr int kDefaultRefill = 4;
 constexpr int kDefaultBoundary = 4;
 constexpr size_t kLdsTableMax = 16384;
-constexpr int kCtrlWords = 8;
-constexpr int kCtrlFirstPass = 6; /* words the first pass resets and reads back */
+/* The control block: ticket counter, records appended, work counters, lane statistics, the
+ * drain's spill counter and resume counter (the words a first pass resets and reads back),
+ * then two words for the math self-test. */
+constexpr int kCtrlQueue = 0, kCtrlRecords = 1, kCtrlCounters = 2;
+constexpr int kCtrlStats = kCtrlCounters + CNT_N;
+constexpr int kCtrlSpill = kCtrlStats + STAT_N;
+constexpr int kCtrlResume = kCtrlSpill + 1;
+constexpr int kCtrlFirstPass = kCtrlResume + 1;
+constexpr int kCtrlWords = 24;
+static_assert(kCtrlFirstPass + 2 <= kCtrlWords, "two words for the math self-test");
+constexpr int kDefaultSpillBelow = 32; /* drain: a wavefront with at most this many photons left spills them */
+constexpr uint64_t kFinalDrain = 16384; /* a drain launch of at most this many photons runs them to the end */
 constexpr size_t kStageBytes = 128u << 20; /* pinned staging buffer for large read-backs */
 
 struct DevBuf {
@@ -55,6 +65,7 @@ struct mcml_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cu_count = 0;
+    int clock_khz = 0; /* the real-time clock the statistics variant stamps */
     char name[256] = {0};
     char err[1024] = {0};
     uint32_t* d_mult = nullptr;
@@ -72,6 +83,7 @@ struct mcml_ctx {
     DevBuf rgrp, rgq;                 /* replay: job range and hand-out counter per (set, SDS) */
     DevBuf wrows, woff, wold, wnew, wout; /* White Monte Carlo inputs and output */
     DevBuf ref_state, ref_active, ref_fibers, ref_spill_key, ref_spill_w; /* reference launch shape */
+    DevBuf spill[2];                  /* drain compaction: photons handed from one launch to the next */
     void* stage[2] = {nullptr, nullptr}; /* pinned read-back staging, allocated on first use */
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     /* allocation-free entry point: this device's scratch array (its device address, taken
@@ -458,6 +470,7 @@ extern "C" mcml_ctx* mcml_create(int device, int* err)
         return nullptr;
     }
     c->cu_count = prop.multiProcessorCount;
+    if (hipDeviceGetAttribute(&c->clock_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->clock_khz = 0;
     snprintf(c->name, sizeof(c->name), "%s", prop.name[0] ? prop.name : prop.gcnArchName);
     if (mcml_scratch_address(&c->scratch) != hipSuccess || c->scratch == nullptr ||
         hipHostMalloc(&c->na_stage, MCML_NOALLOC_OFF_ACC - MCML_NOALLOC_OFF_SETS, hipHostMallocDefault) != hipSuccess ||
@@ -471,8 +484,31 @@ extern "C" mcml_ctx* mcml_create(int device, int* err)
         mcml_destroy(c);
         return nullptr;
     }
+    /* the transport's reciprocal and square root must be the correctly rounded ones on this
+     * device, or photon trajectories would differ from their CPU restatement */
+    uint64_t bad[2] = {0, 0};
+    if (mcml_selftest_math(c, bad) != MCML_OK || bad[0] != 0 || bad[1] != 0) {
+        *err = MCML_ERR_DEVICE;
+        mcml_destroy(c);
+        return nullptr;
+    }
     *err = MCML_OK;
     return c;
+}
+
+extern "C" int mcml_selftest_math(mcml_ctx* c, uint64_t* mismatches)
+{
+    if (!c || !mismatches) return MCML_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    unsigned long long* bad = c->ctrl + kCtrlFirstPass; /* two words past the first pass's */
+    HIP_TRY(c, mcml_launch_reset(bad, 2, c->stream));
+    HIP_TRY(c, mcml_launch_selftest(bad, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->h_ctrl + kCtrlFirstPass, bad, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    mismatches[0] = c->h_ctrl[kCtrlFirstPass];
+    mismatches[1] = c->h_ctrl[kCtrlFirstPass + 1];
+    return MCML_OK;
 }
 
 extern "C" void mcml_destroy(mcml_ctx* c)
@@ -484,7 +520,7 @@ extern "C" void mcml_destroy(mcml_ctx* c)
                       &c->rows,  &c->weights,   &c->precs,   &c->offsets, &c->sums,  &c->rrows, &c->arz,
                       &c->a0z,   &c->wrows,     &c->woff,    &c->wold,    &c->wnew,  &c->wout,
                       &c->ref_state, &c->ref_active, &c->ref_fibers, &c->ref_spill_key, &c->ref_spill_w,
-                      &c->rgrp,      &c->rgq};
+                      &c->rgrp,      &c->rgq,        &c->spill[0],   &c->spill[1]};
     for (DevBuf* b : bufs) release(*b);
     mcml_graph_release(c->graph);
     for (int i = 0; i < 2; i++) {
@@ -597,9 +633,21 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
     a.total = total;
     a.seed = seed;
     a.mult = c->d_mult;
-    a.queue = c->ctrl + 0;
-    a.n_records = c->ctrl + 1;
-    a.counters = c->ctrl + 2;
+    a.queue = c->ctrl + kCtrlQueue;
+    a.n_records = c->ctrl + kCtrlRecords;
+    a.counters = c->ctrl + kCtrlCounters;
+    a.stats = o.lane_stats ? c->ctrl + kCtrlStats : nullptr;
+    /* drain compaction: spill arrays hold one photon per lane of the launch */
+    const int spill_below = o.spill_below < 0 ? 0 : (o.spill_below > 0 ? std::min(o.spill_below, 63) : kDefaultSpillBelow);
+    const uint64_t spill_cap = (uint64_t)grid * (uint64_t)block;
+    const size_t spill_bytes = sizeof(float) * (size_t)SPILL_FIELDS(a.num_layers) * (size_t)spill_cap;
+    if (spill_below > 0) {
+        if ((rc = ensure(c, c->spill[0], spill_bytes)) || (rc = ensure(c, c->spill[1], spill_bytes))) return rc;
+        a.spill = as<float>(c->spill[0]);
+        a.n_spill = c->ctrl + kCtrlSpill;
+        a.spill_cap = spill_cap;
+        a.spill_below = spill_below;
+    }
 
     /* Record capacity.  Each photon yields at most max_overlap records, so `bound` records
      * can never overflow; it is used whenever it fits in a quarter of the free device memory
@@ -632,9 +680,9 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
     cap = std::max<uint64_t>(cap, reserve + 4096);
     if (cap >= bound) cap = std::max<uint64_t>(bound, 1);
     unsigned long long cnt[CNT_N] = {0, 0, 0, 0};
-    double ms_kernel = 0;
     int launches = 0;
     n_rec = 0;
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream)); /* the pass: every launch and the gaps between them */
     for (;;) {
         if ((rc = ensure_keep(c, c->rec_key, sizeof(uint64_t) * (size_t)cap, sizeof(uint64_t) * (size_t)n_rec)) ||
             (rc = ensure_keep(c, c->rec_row, sizeof(float) * (size_t)cap * (size_t)row_stride,
@@ -644,23 +692,21 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
         a.rec_row = as<float>(c->rec_row);
         a.cap = cap;
         a.stop_at = cap >= bound ? ~0ull : cap - reserve;
-        HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-        if (launches == 0 && o.use_graph) {
+        /* a pass that may pause for records does not spill: its spill counter would carry
+         * photons across the pause */
+        if (a.stop_at != ~0ull) a.spill = nullptr;
+        if (launches == 0 && o.use_graph && !o.lane_stats) {
             HIP_TRY(c, mcml_graph_first_pass(c->graph, a, grid, block, c->ctrl, kCtrlFirstPass, c->h_ctrl, c->stream));
-            HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
         } else {
             if (launches == 0) HIP_TRY(c, mcml_launch_reset(c->ctrl, kCtrlFirstPass, c->stream));
             HIP_TRY(c, mcml_launch_first_pass(a, grid, block, c->stream));
-            HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
             HIP_TRY(c, hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(unsigned long long) * kCtrlFirstPass,
                                       hipMemcpyDeviceToHost, c->stream));
         }
         launches++;
         HIP_TRY(c, hipStreamSynchronize(c->stream));
-        ms_kernel += ms_between(c->ev0, c->ev1);
-        const uint64_t claimed = c->h_ctrl[0];
-        n_rec = c->h_ctrl[1];
-        for (int i = 0; i < CNT_N; i++) cnt[i] = c->h_ctrl[2 + i];
+        const uint64_t claimed = c->h_ctrl[kCtrlQueue];
+        n_rec = c->h_ctrl[kCtrlRecords];
         if (n_rec > cap) {
             set_err(c, "first pass appended %llu records into a buffer of %llu", (unsigned long long)n_rec,
                     (unsigned long long)cap);
@@ -677,6 +723,57 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
         uint64_t next = std::max<uint64_t>(2 * cap, (uint64_t)(1.25 * est)) + reserve;
         cap = next >= bound ? std::max<uint64_t>(bound, 1) : next;
     }
+    /* the drain: photons spilled by sparse wavefronts continue in compacted launches; each
+     * spills again while more than kFinalDrain photons are left, the last runs them out */
+    uint64_t n_sp = a.spill != nullptr ? c->h_ctrl[kCtrlSpill] : 0;
+    int drains = 0, from = 0;
+    while (n_sp > 0) {
+        if (n_sp > spill_cap || drains > 64) {
+            set_err(c, "drain: %llu spilled photons for %llu slots after %d launches", (unsigned long long)n_sp,
+                    (unsigned long long)spill_cap, drains);
+            return MCML_ERR_DEVICE;
+        }
+        FirstPassArgs r = a;
+        r.resume = 1;
+        r.resume_buf = as<float>(c->spill[from]);
+        r.resume_cap = spill_cap;
+        r.total = n_sp;
+        r.queue = c->ctrl + kCtrlResume;
+        r.chunk = 64;
+        r.stop_at = ~0ull; /* the record reserve covers every photon that was in flight */
+        const bool again = n_sp > kFinalDrain;
+        r.spill = again ? as<float>(c->spill[from ^ 1]) : nullptr;
+        const int rgrid = (int)std::min<uint64_t>((uint64_t)grid, (n_sp + (uint64_t)block - 1) / (uint64_t)block);
+        HIP_TRY(c, mcml_launch_clear(c->ctrl + kCtrlSpill, 2, c->stream)); /* spill and resume counters */
+        HIP_TRY(c, mcml_launch_first_pass(r, rgrid, block, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(unsigned long long) * kCtrlFirstPass,
+                                  hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        drains++;
+        n_rec = c->h_ctrl[kCtrlRecords];
+        if (n_rec > cap || c->h_ctrl[kCtrlResume] < n_sp) {
+            set_err(c, "drain launch %d: %llu records for %llu slots, %llu of %llu photons taken", drains,
+                    (unsigned long long)n_rec, (unsigned long long)cap, (unsigned long long)c->h_ctrl[kCtrlResume],
+                    (unsigned long long)n_sp);
+            return MCML_ERR_DEVICE;
+        }
+        n_sp = again ? c->h_ctrl[kCtrlSpill] : 0;
+        from ^= 1;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, hipEventSynchronize(c->ev1));
+    const double ms_kernel = ms_between(c->ev0, c->ev1);
+    for (int i = 0; i < CNT_N; i++) cnt[i] = c->h_ctrl[kCtrlCounters + i];
+    if (o.lane_stats) {
+        const unsigned long long* st = c->h_ctrl + kCtrlStats;
+        for (int i = 0; i < 4; i++) out->lane_blocks[i] = st[STAT_STEP + i];
+        const unsigned long long t0 = ~st[STAT_T_START], td = ~st[STAT_T_DRY], te = st[STAT_T_END];
+        if (c->clock_khz > 0 && te >= t0) {
+            out->ms_clock = (double)(te - t0) / c->clock_khz;
+            out->ms_drain = te >= td ? (double)(te - td) / c->clock_khz : 0.0;
+        }
+    }
+    out->drain_launches = drains;
     if (n_rec >= (1ull << 32)) {
         set_err(c, "more than 2^32 detection records");
         return MCML_ERR_LIMIT;
@@ -1221,9 +1318,9 @@ extern "C" int mcml_run_batch_sums_noalloc(mcml_ctx* c, const mcml_sim* sims, in
     a.total = begin[nsets];
     a.seed = seed;
     a.mult = c->d_mult;
-    a.queue = ctrl + 0;
-    a.n_records = ctrl + 1;
-    a.counters = ctrl + 2;
+    a.queue = ctrl + kCtrlQueue;
+    a.n_records = ctrl + kCtrlRecords;
+    a.counters = ctrl + kCtrlCounters;
     a.stop_at = ~0ull;
     a.accum = acc;
     HIP_TRY(c, mcml_launch_clear(ctrl, kCtrlWords, c->stream));

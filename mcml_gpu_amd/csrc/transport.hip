@@ -148,6 +148,34 @@ __device__ __forceinline__ void launch(Photon& p, const DevSet* S, uint64_t seed
     load_layer(p, S);
 }
 
+/* Correctly rounded 1/x and sqrt(x) from the hardware units.  The reciprocal unit's result
+ * (within 1 ulp) refined by one Newton step e = 1 - x y, y + y e (two fmas) is the correctly
+ * rounded 1/x: Markstein's theorem leaves only significands of all ones in doubt, and
+ * mcml_selftest_math checks every significand at 16 exponents on the device.  The square
+ * root unit's result is corrected to the nearest float by testing its two neighbours with
+ * exact fma residuals (the Tuckerman test); below 2^-96, where the residuals underflow, the
+ * compiler's IEEE sequence takes over (mcml_selftest_math checks [2^-126, 4) exhaustively).
+ * So the transport's spin computes exactly 1.0f / x and sqrtf(x), as its CPU restatement
+ * does, in 4 and about 7 issue slots instead of 6 and 11 for Newton iterations from a
+ * bit-pattern seed. */
+__device__ __forceinline__ float rcp_rn(float x)
+{
+    const float y = __builtin_amdgcn_rcpf(x);
+    const float e = fmaf(-x, y, 1.0f);
+    return fmaf(e, y, y);
+}
+
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    if (__builtin_expect(x < 0x1p-96f, 0)) return sqrtf(x);
+    float s = __builtin_amdgcn_sqrtf(x);
+    const uint32_t b = __float_as_uint(s);
+    const float sd = __uint_as_float(b - 1u), su = __uint_as_float(b + 1u);
+    const float rd = fmaf(-sd, s, x), ru = fmaf(-su, s, x);
+    s = rd <= 0.0f ? sd : s;
+    return ru > 0.0f ? su : s;
+}
+
 /* Henyey-Greenstein deflection + azimuth + renormalisation (Spin, transport.cu:742-786),
  * written with one division and one square root on the common path:
  * P = sin(theta) sqrt(1-dz^2), Q = sin(theta) / sqrt(1-dz^2), and a second-order
@@ -174,14 +202,14 @@ __device__ __forceinline__ void spin(Photon& p)
         p.dz = copysignf(cost, p.dz * cost);
     } else {
         /* one division serves the HG inversion (1/D = b r) and the rotation (1/b = D r) */
-        const float r = mcml_rcp(D * b);
+        const float r = rcp_rn(D * b);
         float cost = fmaf(-2.0f, v, 3.0f);
         if (!iso) {
             const float temp = p.hg_a * (b * r);
             cost = fmaf(-temp, temp, p.hg_d) * p.hg_inv2g;
         }
         const float ab = fmaf(-cost, cost, 1.0f) * b;
-        const float P = ab * mcml_rsqrt(ab);
+        const float P = sqrt_rn(ab);
         const float Q = P * (D * r);
         const float cxp = fmaf(Q, fmaf(p.dx * p.dz, cosp, -(p.dy * sinp)), p.dx * cost);
         const float cyp = fmaf(Q, fmaf(p.dy * p.dz, cosp, p.dx * sinp), p.dy * cost);
@@ -462,7 +490,8 @@ __device__ __forceinline__ uint32_t set_of_ticket(const uint64_t* set_begin, int
  * every ticket is run exactly once.  stop_at = ~0 disables the check. */
 struct WaveQueue {
     uint64_t next, end;
-    bool dry;
+    bool dry;       /* no further chunk will be claimed (exhausted or paused) */
+    bool exhausted; /* the counter passed the last ticket */
 };
 
 __device__ __forceinline__ uint64_t take_ids(WaveQueue& q, uint64_t need, bool mine, unsigned long long* counter,
@@ -492,6 +521,7 @@ __device__ __forceinline__ uint64_t take_ids(WaveQueue& q, uint64_t need, bool m
         if (cend > total) cend = total;
         if (base >= total) {
             q.dry = true;
+            q.exhausted = !stop;
             cend = base;
         } else {
             claimed = base;
@@ -567,7 +597,7 @@ __device__ __forceinline__ void acc_weight(unsigned long long* e, float w)
  * transport.cu:341-471, 722-739, 299-311).  PL = also accumulate per-layer pathlength and
  * step count, so that each detection record is already the replay's output row.  SUMS =
  * no records: each detection is added to its (set, SDS) group's accumulators instead. */
-template <bool PL, bool SUMS>
+template <bool PL, bool SUMS, bool STATS = false>
 __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned char* lds_raw)
 {
     const DevSet* sets = a.sets;
@@ -589,7 +619,7 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
     int pend_layer = 0;
     uint32_t set = 0;
     uint64_t key = 0; /* set << 45 | photon index */
-    WaveQueue q = {0, 0, false};
+    WaveQueue q = {0, 0, false, false};
     uint32_t cset = 0;
     uint64_t clo = 0, chi = 0;
     /* per-lane work counters (statistics only; the results do not depend on them).  A
@@ -599,6 +629,8 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
     uint32_t n_int = 0;
     const int refill_min = a.refill_min < 1 ? 1 : a.refill_min;
     const int bnd_min = a.bnd_min < 1 ? 1 : a.bnd_min;
+    uint32_t s_step = 0, s_pend = 0, s_refill = 0, s_dry = 0; /* lane-blocks by state (STATS) */
+    if (STATS && lane_id() == 0) atomicMax(&a.stats[STAT_T_START], ~(unsigned long long)wall_clock64());
 
     for (;;) {
         const uint64_t dead = __ballot(!alive);
@@ -607,12 +639,40 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
             uint64_t claimed;
             const uint64_t t = take_ids(q, dead, !alive, a.queue, (uint64_t)a.chunk, a.total, claimed, a.n_records,
                                         a.stop_at);
+            if (STATS && q.dry && lane_id() == 0) /* this wavefront found the queue dry */
+                atomicMax(&a.stats[STAT_T_DRY], ~(unsigned long long)wall_clock64());
             if (claimed != ~0ull && a.nsets > 1) { /* wave-uniform: cache the new chunk's set */
                 cset = set_of_ticket(a.set_begin, a.nsets, claimed);
                 clo = a.set_begin[cset];
                 chi = a.set_begin[cset + 1];
             }
-            if (t != ~0ull) {
+            if (t != ~0ull && a.resume) { /* continue a spilled photon */
+                const float* col = a.resume_buf + t;
+                const uint64_t cs = a.resume_cap;
+                p.x = col[SP_X * cs];
+                p.y = col[SP_Y * cs];
+                p.z = col[SP_Z * cs];
+                p.dx = col[SP_DX * cs];
+                p.dy = col[SP_DY * cs];
+                p.dz = col[SP_DZ * cs];
+                p.w = col[SP_W * cs];
+                p.layer = __float_as_int(col[SP_LAYER * cs]);
+                p.steps = __float_as_int(col[SP_STEPS * cs]);
+                p.first = __float_as_int(col[SP_FIRST * cs]);
+                p.rng.s = ((uint64_t)__float_as_uint(col[SP_RNG_HI * cs]) << 32) | __float_as_uint(col[SP_RNG_LO * cs]);
+                p.rng.a = __float_as_uint(col[SP_RNG_A * cs]);
+                p.pl_cur = col[SP_PL_CUR * cs];
+                p.pl_layer = __float_as_int(col[SP_PL_LAYER * cs]);
+                key = ((uint64_t)__float_as_uint(col[SP_KEY_HI * cs]) << 32) | __float_as_uint(col[SP_KEY_LO * cs]);
+                const uint32_t pb = __float_as_uint(col[SP_PENDING * cs]);
+                pending = (pb & 1u) != 0u;
+                pend_layer = (int)(pb >> 1);
+                if (PL)
+                    for (int l = 0; l < L; l++) pl_lane[l * pl_stride] = col[(SP_PL0 + l) * cs];
+                set = (uint32_t)(key >> MCML_KEY_SET_SHIFT);
+                load_layer(p, sets + set);
+                alive = true;
+            } else if (t != ~0ull) {
                 set = set_for(a.set_begin, a.nsets, t, cset, clo, chi);
                 const uint64_t k = t - a.set_begin[set];
                 key = ((uint64_t)set << MCML_KEY_SET_SHIFT) | k;
@@ -624,6 +684,49 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
             }
         }
         if (__ballot(alive) == 0) break;
+        if (a.spill != nullptr && q.exhausted) { /* wave-uniform */
+            const uint64_t live = __ballot(alive);
+            if (__popcll(live) <= a.spill_below) { /* spill the remaining photons and stop */
+                unsigned long long base = 0;
+                if (lane_id() == (uint32_t)(__ffsll((long long)__ballot(1)) - 1))
+                    base = atomicAdd(a.n_spill, (unsigned long long)__popcll(live));
+                base = bcast_first(base);
+                if (alive) {
+                    float* col = a.spill + base + rank_in(live);
+                    const uint64_t cs = a.spill_cap;
+                    col[SP_X * cs] = p.x;
+                    col[SP_Y * cs] = p.y;
+                    col[SP_Z * cs] = p.z;
+                    col[SP_DX * cs] = p.dx;
+                    col[SP_DY * cs] = p.dy;
+                    col[SP_DZ * cs] = p.dz;
+                    col[SP_W * cs] = p.w;
+                    col[SP_LAYER * cs] = __int_as_float(p.layer);
+                    col[SP_STEPS * cs] = __int_as_float(p.steps);
+                    col[SP_FIRST * cs] = __int_as_float(p.first);
+                    col[SP_RNG_LO * cs] = __uint_as_float((uint32_t)p.rng.s);
+                    col[SP_RNG_HI * cs] = __uint_as_float((uint32_t)(p.rng.s >> 32));
+                    col[SP_RNG_A * cs] = __uint_as_float(p.rng.a);
+                    col[SP_PL_CUR * cs] = p.pl_cur;
+                    col[SP_PL_LAYER * cs] = __int_as_float(p.pl_layer);
+                    col[SP_KEY_LO * cs] = __uint_as_float((uint32_t)key);
+                    col[SP_KEY_HI * cs] = __uint_as_float((uint32_t)(key >> 32));
+                    col[SP_PENDING * cs] = __uint_as_float((pending ? 1u : 0u) | ((uint32_t)pend_layer << 1));
+                    if (PL)
+                        for (int l = 0; l < L; l++) col[(SP_PL0 + l) * cs] = pl_lane[l * pl_stride];
+                    c_int += n_int; /* the photon's interactions so far; its steps count where it ends */
+                    n_int = 0;
+                    alive = false;
+                }
+                break;
+            }
+        }
+        if (STATS) {
+            s_step += (alive && !pending) ? 1u : 0u;
+            s_pend += (alive && pending) ? 1u : 0u;
+            s_refill += (!alive && !q.dry) ? 1u : 0u;
+            s_dry += (!alive && q.dry) ? 1u : 0u;
+        }
         int st = ST_ALIVE;
         uint32_t mask = 0;
         /* several steps between wavefront-level decisions: a lane that reaches a boundary
@@ -697,6 +800,16 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
         atomicAdd(&a.counters[CNT_BOUNDARY], (unsigned long long)s2);
         atomicAdd(&a.counters[CNT_INTERACTION], (unsigned long long)s3);
     }
+    if (STATS) {
+        const uint64_t b0 = wave_sum(s_step), b1 = wave_sum(s_pend), b2 = wave_sum(s_refill), b3 = wave_sum(s_dry);
+        if (lane_id() == 0) {
+            atomicAdd(&a.stats[STAT_STEP], (unsigned long long)b0);
+            atomicAdd(&a.stats[STAT_PENDING], (unsigned long long)b1);
+            atomicAdd(&a.stats[STAT_REFILL], (unsigned long long)b2);
+            atomicAdd(&a.stats[STAT_DRY], (unsigned long long)b3);
+            atomicMax(&a.stats[STAT_T_END], (unsigned long long)wall_clock64());
+        }
+    }
 }
 
 } // namespace
@@ -712,6 +825,22 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass(FirstPassArgs a)
     stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
     first_pass_body<false, false>(a, lds_raw);
+}
+
+/* Statistics variants: the same passes counting lane-blocks by state and stamping the
+ * real-time clock at the start, when the queue first runs dry and at the end. */
+extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_stats(FirstPassArgs a)
+{
+    stage_tables();
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    first_pass_body<false, false, true>(a, lds_raw);
+}
+
+extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_pl_stats(FirstPassArgs a)
+{
+    stage_tables();
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    first_pass_body<true, false, true>(a, lds_raw);
 }
 
 /* The same pass adding each detection to its group's accumulators (no records). */
@@ -1147,6 +1276,26 @@ extern "C" __global__ void __launch_bounds__(256) mcml_clear_words(unsigned long
         p[i] = 0ull;
 }
 
+/* Self-test of rcp_rn and sqrt_rn against IEEE division and square root: kind 0 checks
+ * every significand of x at 16 exponents from 2^-126 to 2^125, kind 1 every float in
+ * [2^-126, 4).  Mismatches are counted in *bad. */
+extern "C" __global__ void __launch_bounds__(256) mcml_selftest_kernel(int kind, unsigned long long* bad)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool same = true;
+    if (kind == 0) {
+        if (i >= (16ull << 23)) return;
+        const int exps[16] = {-126, -100, -60, -30, -10, -2, -1, 0, 1, 2, 10, 30, 60, 100, 120, 125};
+        const float x = __uint_as_float(((uint32_t)(exps[i >> 23] + 127) << 23) | (uint32_t)(i & 0x7FFFFFu));
+        same = __float_as_uint(rcp_rn(x)) == __float_as_uint(1.0f / x);
+    } else {
+        if (i >= 0x40800000ull - 0x00800000ull) return;
+        const float x = __uint_as_float((uint32_t)(0x00800000ull + i));
+        same = __float_as_uint(sqrt_rn(x)) == __float_as_uint(sqrtf(x));
+    }
+    if (!same) atomicAdd(bad, 1ull);
+}
+
 /* ----------------------------------------------------------------- counters reset */
 /* Zero the control block [queue, n_records, counters...] with a kernel rather than a
  * memset, so the same sequence can be recorded in a HIP graph. */
@@ -1243,6 +1392,16 @@ hipError_t mcml_scratch_address(void** p)
     return hipGetSymbolAddress(p, HIP_SYMBOL(scratch));
 }
 
+hipError_t mcml_launch_selftest(unsigned long long* bad, hipStream_t st)
+{
+    hipLaunchKernelGGL(mcml_selftest_kernel, dim3((unsigned)((16ull << 23) / 256)), dim3(256), 0, st, 0, bad);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mcml_selftest_kernel, dim3((unsigned)((0x40800000ull - 0x00800000ull + 255) / 256)), dim3(256), 0,
+                       st, 1, bad + 1);
+    return hipGetLastError();
+}
+
 hipError_t mcml_launch_first_pass_sums(const FirstPassArgs& a, int grid, int block, hipStream_t st)
 {
     const size_t lds = first_pass_lds(a, block);
@@ -1274,8 +1433,14 @@ hipError_t mcml_launch_sums_finalize(const unsigned long long* accum, int groups
 hipError_t mcml_launch_first_pass(const FirstPassArgs& a, int grid, int block, hipStream_t st)
 {
     const size_t lds = first_pass_lds(a, block);
-    if (a.row_stride > 1) hipLaunchKernelGGL(mcml_first_pass_pl, dim3(grid), dim3(block), lds, st, a);
-    else hipLaunchKernelGGL(mcml_first_pass, dim3(grid), dim3(block), lds, st, a);
+    if (a.stats != nullptr) {
+        if (a.row_stride > 1) hipLaunchKernelGGL(mcml_first_pass_pl_stats, dim3(grid), dim3(block), lds, st, a);
+        else hipLaunchKernelGGL(mcml_first_pass_stats, dim3(grid), dim3(block), lds, st, a);
+    } else if (a.row_stride > 1) {
+        hipLaunchKernelGGL(mcml_first_pass_pl, dim3(grid), dim3(block), lds, st, a);
+    } else {
+        hipLaunchKernelGGL(mcml_first_pass, dim3(grid), dim3(block), lds, st, a);
+    }
     return hipGetLastError();
 }
 

@@ -158,7 +158,8 @@ void oracle_launch(const mcml_derived* d, uint64_t seed, uint32_t set, uint64_t 
  *                                                           sin(theta) / sqrt(1-dz^2))
  *   with one reciprocal r = 1 / (D (1 - dz^2)), D = 1 - g + 2gu, for both divisions
  *   (D formed from v = 2 - u, the float 1.k the uniform is built from):
- *   t = (1 - g^2) / D = (1 - g^2)((1 - dz^2) r) and Q = P (D r); P = x rsqrt(x)
+ *   t = (1 - g^2) / D = (1 - g^2)((1 - dz^2) r) and Q = P (D r); P = sqrtf(x), the
+ *   reciprocal and the square root both correctly rounded
  *   renormalisation by 1 - d/2 + 3d^2/8, d = |u|^2 - 1     (second-order rsqrt about 1;
  *                                                           |d| stays at a few ulp)
  * Returns cos(theta) for the tests. */
@@ -185,14 +186,14 @@ float oracle_spin(oracle_photon* p, float g)
         p->dz = copysignf(cost, p->dz * cost);
     } else {
         /* one division serves the HG inversion (1/D = b r) and the rotation (1/b = D r) */
-        float r = mcml_rcp(D * b);
+        float r = 1.0f / (D * b);
         cost = fmaf(-2.0f, v, 3.0f);
         if (!iso) {
             float temp = (1.0f - g * g) * (b * r);
             cost = fmaf(-temp, temp, 1.0f + g * g) * (1.0f / (2.0f * g));
         }
         float ab = fmaf(-cost, cost, 1.0f) * b;
-        float P = ab * mcml_rsqrt(ab);
+        float P = sqrtf(ab);
         float Q = P * (D * r);
         float cxp = fmaf(Q, fmaf(p->dx * p->dz, cosp, -(p->dy * sinp)), p->dx * cost);
         float cyp = fmaf(Q, fmaf(p->dy * p->dz, cosp, p->dx * sinp), p->dy * cost);
@@ -588,16 +589,6 @@ void oracle_sincos_turns_n(const float* t, float* s, float* c, int64_t n)
 {
     for (int64_t i = 0; i < n; i++) mcml_sincos_turns(t[i], &s[i], &c[i]);
 }
-void oracle_rcp_n(const float* x, float* y, int64_t n)
-{
-    for (int64_t i = 0; i < n; i++) y[i] = mcml_rcp(x[i]);
-}
-
-void oracle_rsqrt_n(const float* x, float* y, int64_t n)
-{
-    for (int64_t i = 0; i < n; i++) y[i] = mcml_rsqrt(x[i]);
-}
-
 void oracle_acos_n(const float* x, float* y, int64_t n)
 {
     for (int64_t i = 0; i < n; i++) y[i] = mcml_acos(x[i]);

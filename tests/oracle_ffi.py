@@ -50,8 +50,6 @@ def lib():
                                 C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_int)]
     L.oracle_sincos_turns_n.argtypes = [_fp, _fp, _fp, C.c_int64]
     L.oracle_acos_n.argtypes = [_fp, _fp, C.c_int64]
-    L.oracle_rcp_n.argtypes = [_fp, _fp, C.c_int64]
-    L.oracle_rsqrt_n.argtypes = [_fp, _fp, C.c_int64]
     L.oracle_spin_n.argtypes = [C.c_float, _fp, C.c_uint64, _fp, _fp, C.c_int64]
     L.oracle_reflect_fraction.argtypes = [D, C.c_int, C.c_int, _fp, C.c_uint64, C.c_int64, _fp]
     L.oracle_reflect_fraction.restype = C.c_double

@@ -110,33 +110,3 @@ def test_acos():
     yb = np.zeros_like(bad)
     O.lib().oracle_acos_n(bad, yb, len(bad))
     assert np.isnan(yb).all()
-
-
-def test_rcp_within_half_ulp():
-    """mcml_rcp (seed + three Newton steps) against float64 1/x over the ranges the spin uses
-    (1 - dz^2 down to 2^-23 times the HG denominator) and across binades."""
-    rng = np.random.default_rng(5)
-    x = np.concatenate([rng.uniform(1e-8, 2.5, 2000000), 10.0 ** rng.uniform(-30, 30, 500000),
-                        2.0 ** np.arange(-60, 60), np.nextafter(2.0 ** np.arange(-60, 60), 0)]).astype(np.float32)
-    y = np.zeros_like(x)
-    O.lib().oracle_rcp_n(x, y, len(x))
-    assert ulp_err(y, 1.0 / x.astype(np.float64)).max() < 0.52
-
-
-def test_rsqrt_within_one_ulp_and_square_root():
-    """mcml_rsqrt (seed + three Newton steps) against float64 1/sqrt(x); x * rsqrt(x) as the
-    square root the spin uses, within 2 ulp; x = 0 gives 0 for that square root."""
-    rng = np.random.default_rng(6)
-    x = np.concatenate([rng.uniform(0, 1, 2000000), 10.0 ** rng.uniform(-30, 30, 500000),
-                        2.0 ** np.arange(-120, 120), np.nextafter(2.0 ** np.arange(-120, 120), 0)]).astype(np.float32)
-    y = np.zeros_like(x)
-    O.lib().oracle_rsqrt_n(x, y, len(x))
-    ref = 1.0 / np.sqrt(x.astype(np.float64))
-    assert ulp_err(y, ref).max() < 1.0
-    s = (x * y).astype(np.float32)
-    sref = np.sqrt(x.astype(np.float64))
-    assert ulp_err(s, sref).max() < 2.0
-    z = np.zeros(1, np.float32)
-    yz = np.zeros(1, np.float32)
-    O.lib().oracle_rsqrt_n(z, yz, 1)
-    assert np.isfinite(yz[0]) and z[0] * yz[0] == 0.0

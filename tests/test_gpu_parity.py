@@ -291,3 +291,54 @@ def test_record_buffer_pause_and_resume(ctx, pathlength):
     assert small.steps == int(cnt[0])
     got = photon_order(small.records)
     assert np.array_equal(got["key"], rec["key"]) and _same_floats(got["weight"], rec["weight"])
+
+
+@pytest.mark.gpu
+def test_lane_statistics_variant_same_results(ctx):
+    """The statistics variant of the first pass returns the same records and counters, and
+    lane-blocks that account for every stepped photon step."""
+    sim = F.ring_busy(200000)
+    opts = dict(replay=True, pathlength=True)
+    ref = ctx.run(sim, make_opts(**opts), seed=SEED)
+    st = ctx.run(sim, make_opts(lane_stats=True, **opts), seed=SEED)
+    assert np.array_equal(st.records, ref.records) and st.steps == ref.steps
+    assert np.array_equal(_bits(st.pathlength), _bits(ref.pathlength))
+    lb = st.lane_blocks.astype(np.int64)
+    print("lane blocks", lb, "drain ms", st.ms_drain, "clock ms", st.ms_clock, flush=True)
+    assert lb[0] > 0 and lb[0] * 12 >= st.steps  # a stepping lane takes at most 12 steps per block
+    assert 0 < st.ms_drain < st.ms_clock
+    assert ref.lane_blocks.sum() == 0 and ref.ms_drain == 0
+
+
+@pytest.mark.gpu
+def test_device_reciprocal_and_square_root_are_correctly_rounded(ctx):
+    """The spin's reciprocal and square root (hardware units plus one correction) equal IEEE
+    1.0f / x and sqrtf(x) on every value the self-test covers (about 1.2e9)."""
+    bad = (C.c_uint64 * 2)()
+    assert ctx.lib.mcml_selftest_math(ctx.h, bad) == 0
+    assert list(bad) == [0, 0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ring_busy", "max_stack_fiber"])
+def test_drain_compaction_does_not_change_results(ctx, name):
+    """Photons handed from sparse wavefronts to compacted follow-up launches continue exactly
+    where they stopped: records, pathlength rows and step counts equal those of a pass with
+    the drain compaction off, and the oracle's."""
+    sim = CASES[name]()
+    sim.number_photons = 600000 if name == "ring_busy" else 200000
+    opts = dict(replay=True, pathlength=True)
+    off = ctx.run(sim, make_opts(spill_below=-1, **opts), seed=SEED)
+    assert off.drain_launches == 0
+    for sb in (0, 63):
+        on = ctx.run(sim, make_opts(spill_below=sb, **opts), seed=SEED)
+        print(name, "spill_below", sb, "drain launches", on.drain_launches, flush=True)
+        assert on.drain_launches >= 1
+        assert np.array_equal(on.records, off.records)
+        assert (on.photons, on.steps, on.steps_boundary, on.steps_interaction) == \
+            (off.photons, off.steps, off.steps_boundary, off.steps_interaction)
+        assert np.array_equal(_bits(on.pathlength), _bits(off.pathlength))
+    rec, cnt = O.first_pass(O.derive(sim), SEED, 0, sim.number_photons)
+    assert off.steps == int(cnt[0])
+    got = photon_order(on.records)
+    assert np.array_equal(got["key"], rec["key"]) and _same_floats(got["weight"], rec["weight"])
