@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--bnd", type=int, default=0)
     ap.add_argument("--spill-below", type=int, default=0,
                     help="drain compaction threshold (0 = library default, -1 = off)")
+    ap.add_argument("--separate-replay", action="store_true",
+                    help="-A: replay in a pass of its own after the first pass instead of within it")
     ap.add_argument("--pathlength", action="store_true",
                     help="cfg3: also record per-layer pathlengths and dump the records (-R -P)")
     ap.add_argument("--absorbance", action="store_true",
@@ -121,7 +123,7 @@ def main():
     absorb = args.config == "cfg2" or args.absorbance
     opts = M.make_opts(replay=True, pathlength=not absorb, absorbance=absorb, binary=True, block_size=args.block,
                        chunk=args.chunk, refill_min=args.refill, blocks_per_cu=args.bpc,
-                       boundary_min=args.bnd, spill_below=args.spill_below)
+                       boundary_min=args.bnd, spill_below=args.spill_below, separate_replay=args.separate_replay)
     flags = "-R -A -B" if absorb else "-R -P -B"
     ctx = M.Context(0)
     for _ in range(args.warmup):

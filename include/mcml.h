@@ -159,6 +159,10 @@ typedef struct {
     int32_t spill_below;     /* drain compaction: once the photon queue is exhausted, a
                                 wavefront left with at most this many photons hands them to a
                                 compacted follow-up launch; 0 = default (32), -1 = off */
+    int32_t separate_replay; /* -A: 1 = replay the detected photons in a pass of their own after
+                                the first pass (as the reference does) instead of within it.
+                                Also taken when the grids exceed 4 GiB or 65535 bins a side,
+                                and with lane_stats or reference_shape */
 } mcml_opts;
 
 /* First-pass detection record: photon `key` = (set << 40) | photon index, detected by

@@ -98,10 +98,12 @@ def load_config(name):
 
 def make_opts(replay=False, absorbance=False, pathlength=False, avg_pathlength=False, binary=False,
               block_size=0, blocks_per_cu=0, chunk=0, refill_min=0, boundary_min=0, use_graph=False,
-              skip_records=False, reference_shape=False, record_cap=0, lane_stats=False, spill_below=0):
+              skip_records=False, reference_shape=False, record_cap=0, lane_stats=False, spill_below=0,
+              separate_replay=False):
     return abi.Opts(int(replay), int(absorbance), int(pathlength), int(avg_pathlength), int(binary),
                     block_size, blocks_per_cu, chunk, refill_min, boundary_min, int(reference_shape),
-                    int(use_graph), int(skip_records), int(record_cap), int(lane_stats), int(spill_below))
+                    int(use_graph), int(skip_records), int(record_cap), int(lane_stats), int(spill_below),
+                    int(separate_replay))
 
 
 class _ResultOwner:

@@ -95,7 +95,7 @@ class Opts(C.Structure):
         "replay", "absorbance", "pathlength", "avg_pathlength", "binary",
         "block_size", "blocks_per_cu", "chunk", "refill_min", "boundary_min", "reference_shape",
         "use_graph", "skip_records")] + [("record_cap", C.c_uint64), ("lane_stats", C.c_int32),
-                                         ("spill_below", C.c_int32)]
+                                         ("spill_below", C.c_int32), ("separate_replay", C.c_int32)]
 
 
 class Record(C.Structure):

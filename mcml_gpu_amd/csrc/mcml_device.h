@@ -101,6 +101,19 @@ struct FirstPassArgs {
     uint64_t resume_cap;           /* column stride of resume_buf */
     int32_t spill_below;
     int32_t resume;
+    /* Absorbance replays in the drain launch (mcml_first_pass_grids; NULL otherwise).  Its
+     * tickets past the resume_photons spilled photons are replay jobs, the record keys
+     * replay_keys[t - resume_photons] of the main launch; a lane runs the job's photon again
+     * for its detector, and a lane whose own photon is detected replays it for each detector
+     * that recorded it, booking the drops into that (set, SDS) group's slabs of nz*nr / nz
+     * cells. */
+    const uint64_t* replay_keys;
+    uint64_t resume_photons;
+    unsigned long long* A_rz;
+    unsigned long long* A0_z;
+    unsigned long long* replay_missed; /* replays that ended without the detection (never expected) */
+    int32_t nr, nz;
+    float rdr, rdz;                    /* 1/dr, 1/dz: bins are trunc(r * rdr), trunc(z * rdz) */
 };
 
 /* Fields of one spilled photon (each a 32-bit column of the SoA spill arrays): position,
@@ -142,6 +155,7 @@ struct ReplayArgs {
     unsigned long long* A0_z;
     int32_t num_sds, nr, nz, pad;
     float rdr, rdz;            /* 1/dr, 1/dz: bins are trunc(r * rdr), trunc(z * rdz) */
+    unsigned long long* missed; /* jobs whose replay ended without the detection (never expected) */
 };
 
 /* Reference launch shape (transport.cu:57-147, 341-471): one photon per thread, at most

@@ -46,7 +46,8 @@ constexpr int kCtrlQueue = 0, kCtrlRecords = 1, kCtrlCounters = 2;
 constexpr int kCtrlStats = kCtrlCounters + CNT_N;
 constexpr int kCtrlSpill = kCtrlStats + STAT_N;
 constexpr int kCtrlResume = kCtrlSpill + 1;
-constexpr int kCtrlFirstPass = kCtrlResume + 1;
+constexpr int kCtrlMissed = kCtrlResume + 1; /* replays that ended without their detection */
+constexpr int kCtrlFirstPass = kCtrlMissed + 1;
 constexpr int kCtrlWords = 24;
 static_assert(kCtrlFirstPass + 2 <= kCtrlWords, "two words for the math self-test");
 constexpr int kDefaultSpillBelow = 32; /* drain: a wavefront with at most this many photons left spills them */
@@ -587,8 +588,16 @@ int validate_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, std::vector<mcm
 }
 
 /* First pass: leaves n_rec records in c->rec_key / c->rec_row (row_stride floats each). */
+/* Absorbance grids the first pass books its replays into (-R -A). */
+struct GridSpec {
+    unsigned long long* A_rz;
+    unsigned long long* A0_z;
+    int nr, nz;
+    float rdr, rdz;
+};
+
 int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opts& o, uint64_t seed, int row_stride,
-               mcml_result* out, uint64_t& n_rec)
+               mcml_result* out, uint64_t& n_rec, const GridSpec* grids = nullptr, bool* grids_done = nullptr)
 {
     const int nsets = (int)der.size();
     std::vector<DevSet> hs((size_t)nsets);
@@ -724,9 +733,50 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
         cap = next >= bound ? std::max<uint64_t>(bound, 1) : next;
     }
     /* the drain: photons spilled by sparse wavefronts continue in compacted launches; each
-     * spills again while more than kFinalDrain photons are left, the last runs them out */
+     * spills again while more than kFinalDrain photons are left, the last runs them out.
+     * With absorbance grids there is one drain launch, run to the end, which also replays
+     * every detection of the main launch into the grids (and its own as they happen), so
+     * that the replays of a few long trajectories overlap the drain rather than follow it.
+     * Many detections (more than the launch has lanes) are left to the replay pass, whose
+     * LDS tiles absorb their drops. */
     uint64_t n_sp = a.spill != nullptr ? c->h_ctrl[kCtrlSpill] : 0;
     int drains = 0, from = 0;
+    if (grids) *grids_done = n_rec <= (uint64_t)grid * (uint64_t)block;
+    if (grids && *grids_done && n_sp + n_rec > 0) {
+        FirstPassArgs r = a;
+        r.resume = 1;
+        r.resume_buf = as<float>(c->spill[0]);
+        r.resume_cap = spill_cap;
+        r.resume_photons = n_sp;
+        r.replay_keys = as<uint64_t>(c->rec_key);
+        r.total = n_sp + n_rec;
+        r.queue = c->ctrl + kCtrlResume;
+        r.chunk = 64;
+        r.stop_at = ~0ull;
+        r.spill = nullptr;
+        r.A_rz = grids->A_rz;
+        r.A0_z = grids->A0_z;
+        r.nr = grids->nr;
+        r.nz = grids->nz;
+        r.rdr = grids->rdr;
+        r.rdz = grids->rdz;
+        r.replay_missed = c->ctrl + kCtrlMissed;
+        const int rgrid = (int)std::min<uint64_t>((uint64_t)grid, (r.total + (uint64_t)block - 1) / (uint64_t)block);
+        HIP_TRY(c, mcml_launch_clear(c->ctrl + kCtrlSpill, 2, c->stream)); /* spill and resume counters */
+        HIP_TRY(c, mcml_launch_first_pass(r, rgrid, block, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(unsigned long long) * kCtrlFirstPass,
+                                  hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        drains++;
+        n_rec = c->h_ctrl[kCtrlRecords];
+        if (n_rec > cap || c->h_ctrl[kCtrlResume] < r.total) {
+            set_err(c, "replay launch: %llu records for %llu slots, %llu of %llu tickets taken",
+                    (unsigned long long)n_rec, (unsigned long long)cap, (unsigned long long)c->h_ctrl[kCtrlResume],
+                    (unsigned long long)r.total);
+            return MCML_ERR_DEVICE;
+        }
+        n_sp = 0;
+    }
     while (n_sp > 0) {
         if (n_sp > spill_cap || drains > 64) {
             set_err(c, "drain: %llu spilled photons for %llu slots after %d launches", (unsigned long long)n_sp,
@@ -763,6 +813,10 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     HIP_TRY(c, hipEventSynchronize(c->ev1));
     const double ms_kernel = ms_between(c->ev0, c->ev1);
+    if (grids && *grids_done && c->h_ctrl[kCtrlMissed] != 0) {
+        set_err(c, "%llu replays ended without the detection they replay", (unsigned long long)c->h_ctrl[kCtrlMissed]);
+        return MCML_ERR_DEVICE;
+    }
     for (int i = 0; i < CNT_N; i++) cnt[i] = c->h_ctrl[kCtrlCounters + i];
     if (o.lane_stats) {
         const unsigned long long* st = c->h_ctrl + kCtrlStats;
@@ -1028,6 +1082,61 @@ int gather_and_reduce(mcml_ctx* c, uint64_t n, int nsets, int S, const float* ro
     return MCML_OK;
 }
 
+/* The replay of every detected photon (MCd_replay, transport.cu:473-622) in a pass of its
+ * own: jobs are the sorted record keys, grouped by (set, SDS) at `off`; rows go to c->rrows,
+ * drops to the grids of gs (none when gs.A_rz is NULL).  Buffers rrows, rgrp and rgq must
+ * hold n rows and the groups. */
+int replay_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opts& o, uint64_t seed, uint64_t n,
+                const std::vector<uint64_t>& off, const GridSpec& gs, mcml_result* out)
+{
+    const int S = der[0].num_sds;
+    const size_t groups = der.size() * (size_t)S;
+    HIP_TRY(c, hipMemcpyAsync(c->rgrp.p, off.data(), sizeof(uint64_t) * (groups + 1), hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->rgq.p, 0, sizeof(unsigned long long) * groups, c->stream));
+    ReplayArgs a;
+    memset(&a, 0, sizeof(a));
+    a.sets = as<DevSet>(c->sets);
+    a.keys = as<uint64_t>(c->skey);
+    a.njobs = n;
+    a.seed = seed;
+    a.mult = c->d_mult;
+    a.grp_begin = as<uint64_t>(c->rgrp);
+    a.grp_queue = as<unsigned long long>(c->rgq);
+    a.ngroups = (int32_t)groups;
+    a.chunk = 64;
+    a.num_layers = der[0].num_layers;
+    a.bnd_min = o.boundary_min > 0 ? std::min(o.boundary_min, 64) : kDefaultBoundary;
+    a.rows = as<float>(c->rrows);
+    a.A_rz = gs.A_rz;
+    a.A0_z = gs.A0_z;
+    a.num_sds = S;
+    a.nr = gs.nr;
+    a.nz = gs.nz;
+    a.rdr = gs.rdr;
+    a.rdz = gs.rdz;
+    a.missed = c->ctrl + kCtrlMissed;
+    HIP_TRY(c, mcml_launch_reset(c->ctrl + kCtrlMissed, 1, c->stream));
+    int occ = 0;
+    HIP_TRY(c, mcml_occupancy_replay(&occ));
+    if (occ < 1) occ = 1;
+    const uint64_t per_block = (uint64_t)mcml_replay_block();
+    const uint64_t grid =
+        std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cu_count * occ, (n + per_block - 1) / per_block));
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, mcml_launch_replay(a, (int)grid, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->h_ctrl + kCtrlMissed, c->ctrl + kCtrlMissed, sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    out->ms_replay_kernel = ms_between(c->ev0, c->ev1);
+    if (c->h_ctrl[kCtrlMissed] != 0) {
+        set_err(c, "%llu replays ended without the detection they replay", (unsigned long long)c->h_ctrl[kCtrlMissed]);
+        return MCML_ERR_DEVICE;
+    }
+    return MCML_OK;
+}
+
 } // namespace
 
 extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, const mcml_opts* opts, uint64_t seed,
@@ -1071,17 +1180,36 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
         mcml_result_free(out);
         return MCML_ERR_NOMEM;
     }
-    /* replay without absorbance maps: record the pathlengths in the first pass itself
-     * (the reference launch shape replays, as the reference does) */
-    const bool inline_pl = o.replay && !o.absorbance && !o.reference_shape;
+    /* Replay: record the pathlengths in the first pass itself, and with -A let each lane
+     * replay its detected photons into the grids within the same pass.  The reference launch
+     * shape (and the lane-statistics variant with -A) replays as a separate pass, as the
+     * reference does. */
+    const bool grids_inline = o.replay && o.absorbance && !o.reference_shape && !o.lane_stats && !o.separate_replay &&
+                              nr < 65536 && nz < 65536 && groups * (size_t)nz * (size_t)nr * 8 < (1ull << 32);
+    const bool inline_pl = o.replay && (!o.absorbance || grids_inline) && !o.reference_shape;
     const int stride = inline_pl ? L + 2 : 1;
+    const size_t rz_cells = o.absorbance ? groups * (size_t)nz * (size_t)nr : 0;
+    const size_t z_cells = o.absorbance ? groups * (size_t)nz : 0;
     const double t0 = now_ms();
+    GridSpec gs = {nullptr, nullptr, nr, nz, 1.0f / der[0].grid_dr, 1.0f / der[0].grid_dz};
+    if (grids_inline) {
+        if ((rc = ensure(c, c->arz, sizeof(unsigned long long) * rz_cells)) ||
+            (rc = ensure(c, c->a0z, sizeof(unsigned long long) * z_cells))) {
+            mcml_result_free(out);
+            return rc;
+        }
+        HIP_TRY(c, hipMemsetAsync(c->arz.p, 0, sizeof(unsigned long long) * rz_cells, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->a0z.p, 0, sizeof(unsigned long long) * z_cells, c->stream));
+        gs.A_rz = as<unsigned long long>(c->arz);
+        gs.A0_z = as<unsigned long long>(c->a0z);
+    }
     uint64_t n = 0;
     std::vector<uint64_t> off, host_count;
     std::vector<double> host_sum;
     double dump_ms = 0;
+    bool grids_done = false; /* the first pass replayed into the grids itself */
     rc = o.reference_shape ? first_pass_refshape(c, der[0], o, seed, out, n, host_count, host_sum)
-                           : first_pass(c, der, o, seed, stride, out, n);
+                           : first_pass(c, der, o, seed, stride, out, n, grids_inline ? &gs : nullptr, &grids_done);
     if (rc || (rc = sort_records(c, n)) ||
         (rc = gather_and_reduce(c, n, nsets, S, as<float>(c->rec_row), stride, as<uint32_t>(c->sidx),
                                 inline_pl ? L + 2 : 0, !o.skip_records, out->reflectance, off, &dump_ms))) {
@@ -1124,6 +1252,7 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
             return MCML_ERR_NOMEM;
         }
         memcpy(out->pl_offset, off.data(), sizeof(uint64_t) * (groups + 1));
+        const bool grids = o.absorbance != 0;
         if (inline_pl) { /* the first pass already produced the replay rows, in file order */
             memcpy(out->replay_reflectance, out->reflectance, sizeof(double) * groups);
             out->ms_dump_kernel = dump_ms;
@@ -1131,10 +1260,19 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
                 mcml_result_free(out);
                 return rc;
             }
-        } else { /* absorbance maps need the true replay (transport.cu:160-222) */
-            const bool grids = o.absorbance != 0;
-            const size_t rz_cells = grids ? groups * (size_t)nz * (size_t)nr : 0;
-            const size_t z_cells = grids ? groups * (size_t)nz : 0;
+        }
+        if (grids && inline_pl && !grids_done && n > 0) { /* the replay pass books the grids only */
+            if ((rc = ensure(c, c->rrows, sizeof(float) * (size_t)n * (size_t)(L + 2))) ||
+                (rc = ensure(c, c->rgrp, sizeof(uint64_t) * (groups + 1))) ||
+                (rc = ensure(c, c->rgq, sizeof(unsigned long long) * groups))) {
+                mcml_result_free(out);
+                return rc;
+            }
+            if ((rc = replay_pass(c, der, o, seed, n, off, gs, out))) {
+                mcml_result_free(out);
+                return rc;
+            }
+        } else if (!inline_pl) { /* absorbance maps replayed in a pass of their own (transport.cu:160-222) */
             if ((rc = ensure(c, c->rrows, sizeof(float) * (size_t)n * (size_t)(L + 2))) ||
                 (rc = ensure(c, c->arz, sizeof(unsigned long long) * rz_cells)) ||
                 (rc = ensure(c, c->a0z, sizeof(unsigned long long) * z_cells))) {
@@ -1144,48 +1282,16 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
             if (grids) {
                 HIP_TRY(c, hipMemsetAsync(c->arz.p, 0, sizeof(unsigned long long) * rz_cells, c->stream));
                 HIP_TRY(c, hipMemsetAsync(c->a0z.p, 0, sizeof(unsigned long long) * z_cells, c->stream));
+                gs.A_rz = as<unsigned long long>(c->arz);
+                gs.A0_z = as<unsigned long long>(c->a0z);
             }
             if (n) {
                 if ((rc = ensure(c, c->rgrp, sizeof(uint64_t) * (groups + 1))) ||
-                    (rc = ensure(c, c->rgq, sizeof(unsigned long long) * groups))) {
+                    (rc = ensure(c, c->rgq, sizeof(unsigned long long) * groups)) ||
+                    (rc = replay_pass(c, der, o, seed, n, off, gs, out))) {
                     mcml_result_free(out);
                     return rc;
                 }
-                HIP_TRY(c, hipMemcpyAsync(c->rgrp.p, off.data(), sizeof(uint64_t) * (groups + 1),
-                                          hipMemcpyHostToDevice, c->stream));
-                HIP_TRY(c, hipMemsetAsync(c->rgq.p, 0, sizeof(unsigned long long) * groups, c->stream));
-                ReplayArgs a;
-                memset(&a, 0, sizeof(a));
-                a.sets = as<DevSet>(c->sets);
-                a.keys = as<uint64_t>(c->skey);
-                a.njobs = n;
-                a.seed = seed;
-                a.mult = c->d_mult;
-                a.grp_begin = as<uint64_t>(c->rgrp);
-                a.grp_queue = as<unsigned long long>(c->rgq);
-                a.ngroups = (int32_t)groups;
-                a.chunk = 64;
-                a.num_layers = L;
-                a.bnd_min = o.boundary_min > 0 ? std::min(o.boundary_min, 64) : kDefaultBoundary;
-                a.rows = as<float>(c->rrows);
-                a.A_rz = grids ? as<unsigned long long>(c->arz) : nullptr;
-                a.A0_z = grids ? as<unsigned long long>(c->a0z) : nullptr;
-                a.num_sds = S;
-                a.nr = nr;
-                a.nz = nz;
-                a.rdr = 1.0f / der[0].grid_dr;
-                a.rdz = 1.0f / der[0].grid_dz;
-                int occ = 0;
-                HIP_TRY(c, mcml_occupancy_replay(&occ));
-                if (occ < 1) occ = 1;
-                const uint64_t per_block = (uint64_t)mcml_replay_block();
-                const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cu_count * occ,
-                                                                             (n + per_block - 1) / per_block));
-                HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-                HIP_TRY(c, mcml_launch_replay(a, (int)grid, c->stream));
-                HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
-                HIP_TRY(c, hipStreamSynchronize(c->stream));
-                out->ms_replay_kernel = ms_between(c->ev0, c->ev1);
             }
             std::vector<uint64_t> off2;
             if ((rc = gather_and_reduce(c, n, nsets, S, as<float>(c->rrows), L + 2, nullptr, L + 2, false,
@@ -1197,18 +1303,18 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
                 mcml_result_free(out);
                 return rc;
             }
-            if (grids) {
-                out->A_rz = (uint64_t*)host_alloc(sizeof(uint64_t) * std::max<size_t>(rz_cells, 1));
-                out->A0_z = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(z_cells, 1));
-                if (!out->A_rz || !out->A0_z) {
-                    mcml_result_free(out);
-                    return MCML_ERR_NOMEM;
-                }
-                if ((rc = copy_d2h(c, out->A_rz, c->arz.p, sizeof(uint64_t) * rz_cells)) ||
-                    (rc = copy_d2h(c, out->A0_z, c->a0z.p, sizeof(uint64_t) * z_cells))) {
-                    mcml_result_free(out);
-                    return rc;
-                }
+        }
+        if (grids) {
+            out->A_rz = (uint64_t*)host_alloc(sizeof(uint64_t) * std::max<size_t>(rz_cells, 1));
+            out->A0_z = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(z_cells, 1));
+            if (!out->A_rz || !out->A0_z) {
+                mcml_result_free(out);
+                return MCML_ERR_NOMEM;
+            }
+            if ((rc = copy_d2h(c, out->A_rz, c->arz.p, sizeof(uint64_t) * rz_cells)) ||
+                (rc = copy_d2h(c, out->A0_z, c->a0z.p, sizeof(uint64_t) * z_cells))) {
+                mcml_result_free(out);
+                return rc;
             }
         }
         out->replayed = 1;

@@ -597,7 +597,46 @@ __device__ __forceinline__ void acc_weight(unsigned long long* e, float w)
  * transport.cu:341-471, 722-739, 299-311).  PL = also accumulate per-layer pathlength and
  * step count, so that each detection record is already the replay's output row.  SUMS =
  * no records: each detection is added to its (set, SDS) group's accumulators instead. */
-template <bool PL, bool SUMS, bool STATS = false>
+/* Book one drop of a lane replaying its detected photon for group `grp` (the replay's
+ * bookkeeping of transport.cu:580-600, as in interact<true>): the first drop to A0_z, later
+ * ones to A_rz, merged while consecutive drops stay in one cell.  The run is kept in two
+ * registers: its cell as z bin << 16 | r bin (grids of fewer than 65536 bins a side; ~0 =
+ * none) and its u32 weight, flushed before it would wrap.  Cells are addressed by 32-bit
+ * byte offsets from the grids' base (grids below 4 GiB), one u64 atomic per run. */
+__device__ __forceinline__ void flush_drop_run(const FirstPassArgs& a, uint32_t grp, uint32_t rl_key, uint32_t rl_w)
+{
+    if (rl_w == 0) return;
+    const uint32_t cell = (grp * (uint32_t)a.nz + (rl_key >> 16)) * (uint32_t)a.nr + (rl_key & 0xFFFFu);
+    atomicAdd(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(a.A_rz) + cell * 8u),
+              (unsigned long long)rl_w);
+}
+
+__device__ __forceinline__ void book_drop(const FirstPassArgs& a, const Photon& p, uint32_t grp, float w_temp,
+                                          uint32_t& rl_key, uint32_t& rl_w)
+{
+    float qz = p.z * a.rdz;
+    qz = fminf(fmaxf(qz, 0.0f), (float)(a.nz - 1));
+    const uint32_t zb = (uint32_t)(int)qz;
+    const uint32_t wd = (uint32_t)w_temp; /* a drop is at most the weight, below 2^32 */
+    if (p.first) {
+        const uint32_t cell = grp * (uint32_t)a.nz + zb;
+        atomicAdd(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(a.A0_z) + cell * 8u),
+                  (unsigned long long)wd);
+        return;
+    }
+    float qr = sqrt_rn(p.x * p.x + p.y * p.y) * a.rdr;
+    qr = fminf(fmaxf(qr, 0.0f), (float)(a.nr - 1));
+    const uint32_t k = (zb << 16) | (uint32_t)(int)qr;
+    if (k == rl_key && rl_w + wd >= rl_w) {
+        rl_w += wd;
+        return;
+    }
+    flush_drop_run(a, grp, rl_key, rl_w);
+    rl_key = k;
+    rl_w = wd;
+}
+
+template <bool PL, bool SUMS, bool STATS = false, bool GRIDS = false>
 __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned char* lds_raw)
 {
     const DevSet* sets = a.sets;
@@ -630,6 +669,11 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
     const int refill_min = a.refill_min < 1 ? 1 : a.refill_min;
     const int bnd_min = a.bnd_min < 1 ? 1 : a.bnd_min;
     uint32_t s_step = 0, s_pend = 0, s_refill = 0, s_dry = 0; /* lane-blocks by state (STATS) */
+    /* GRIDS: detectors still to replay the lane's detected photon for (bit i = SDS i; the
+     * lowest is the replay in progress, none = the lane runs its first-pass photon), and the
+     * drop run of the replay */
+    uint32_t rp_mask = 0;
+    uint32_t rl_key = ~0u, rl_w = 0;
     if (STATS && lane_id() == 0) atomicMax(&a.stats[STAT_T_START], ~(unsigned long long)wall_clock64());
 
     for (;;) {
@@ -646,7 +690,16 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                 clo = a.set_begin[cset];
                 chi = a.set_begin[cset + 1];
             }
-            if (t != ~0ull && a.resume) { /* continue a spilled photon */
+            if (GRIDS && t != ~0ull && a.resume && t >= a.resume_photons) { /* a replay job */
+                const uint64_t rk = a.replay_keys[t - a.resume_photons];
+                set = (uint32_t)(rk >> MCML_KEY_SET_SHIFT);
+                key = rk & ~(31ull << MCML_KEY_SDS_SHIFT);
+                launch(p, sets + set, a.seed, set, rk & MCML_KEY_K_MASK, a.mult);
+                rp_mask = 1u << (uint32_t)((rk >> MCML_KEY_SDS_SHIFT) & 31u);
+                rl_key = ~0u;
+                rl_w = 0;
+                alive = true;
+            } else if (t != ~0ull && a.resume) { /* continue a spilled photon */
                 const float* col = a.resume_buf + t;
                 const uint64_t cs = a.resume_cap;
                 p.x = col[SP_X * cs];
@@ -746,18 +799,33 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                 pend_layer = nl;
             }
             if (act && !bnd && s > 0.0f) {
-                n_int++;
-                p.w -= rintf(p.albedo * p.w);
+                const float w_temp = rintf(p.albedo * p.w);
+                p.w -= w_temp;
+                if (GRIDS && rp_mask != 0) {
+                    book_drop(a, p, set * (uint32_t)sets[set].num_sds + (uint32_t)(__ffs(rp_mask) - 2), w_temp,
+                              rl_key, rl_w);
+                    p.first = 0;
+                } else {
+                    n_int++;
+                }
                 spin(p);
             }
             if (act && !bnd) st = survive(p);
         }
         if (resolve_now(pending, alive, bnd_min) && pending) {
             pending = false;
-            c_bnd++;
-            st = cross<PL>(p, pend_layer, sets + set, 0, mask, pl_lane, pl_stride);
+            if (!GRIDS || rp_mask == 0) c_bnd++;
+            st = cross<PL>(p, pend_layer, sets + set, GRIDS && rp_mask != 0 ? __ffs(rp_mask) - 1 : 0, mask, pl_lane,
+                           pl_stride);
         }
-        if (st != ST_ALIVE) {
+        if (GRIDS && rp_mask != 0 && st != ST_ALIVE) { /* a replay ended: flush its run */
+            const int sds = __ffs(rp_mask) - 1;
+            flush_drop_run(a, set * (uint32_t)sets[set].num_sds + (uint32_t)(sds - 1), rl_key, rl_w);
+            if (st != ST_DETECTED) atomicAdd(a.replay_missed, 1ull);
+            rp_mask &= rp_mask - 1u; /* done with this detector */
+            alive = false;
+        } else if (st != ST_ALIVE) {
+            if (GRIDS && st == ST_DETECTED) rp_mask = mask;
             alive = false;
             if (PL) pl_lane[(p.pl_layer - 1) * pl_stride] = p.pl_cur;
             c_steps += (uint32_t)p.steps;
@@ -791,6 +859,12 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                     idx++;
                 }
             }
+        }
+        if (GRIDS && st != ST_ALIVE && rp_mask != 0) { /* replay the detected photon for its next detector */
+            launch(p, sets + set, a.seed, set, key & MCML_KEY_K_MASK, a.mult);
+            rl_key = ~0u;
+            rl_w = 0;
+            alive = true;
         }
     }
     const uint64_t s0 = wave_sum(c_photons), s1 = wave_sum(c_steps), s2 = wave_sum(c_bnd), s3 = wave_sum(c_int);
@@ -841,6 +915,17 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_pl_stats(FirstPassAr
     stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
     first_pass_body<true, false, true>(a, lds_raw);
+}
+
+/* The drain launch of a -R -A pass: it continues the main launch's spilled photons, takes the
+ * main launch's detections as replay jobs into the absorbance grids, and replays its own
+ * detections in the lanes that made them (records with their pathlength rows as in
+ * mcml_first_pass_pl). */
+extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_grids(FirstPassArgs a)
+{
+    stage_tables();
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    first_pass_body<true, false, false, true>(a, lds_raw);
 }
 
 /* The same pass adding each detection to its group's accumulators (no records). */
@@ -1134,6 +1219,7 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
             pl_lane[(p.pl_layer - 1) * REPLAY_BLOCK] = p.pl_cur;
             if (gl.a_rz != nullptr) flush_run(gl, a.nr);
             float* row = a.rows + job * (uint64_t)(L + 2);
+            if (st != ST_DETECTED) atomicAdd(a.missed, 1ull);
             row[0] = (st == ST_DETECTED) ? credited(p, S, (int)jsds) : 0.0f;
             for (int l = 0; l < L; l++) row[1 + l] = pl_lane[l * REPLAY_BLOCK];
             row[L + 1] = (float)p.steps;
@@ -1334,7 +1420,8 @@ hipError_t mcml_graph_first_pass(FirstPassGraph& G, const FirstPassArgs& a, int 
     void* fp_params[] = {(void*)&args};
     void* rs_params[] = {(void*)&ctrl_arg, (void*)&n_arg};
     hipKernelNodeParams kp = {};
-    kp.func = a.row_stride > 1 ? (void*)mcml_first_pass_pl : (void*)mcml_first_pass;
+    kp.func = a.A_rz != nullptr ? (void*)mcml_first_pass_grids
+                                : (a.row_stride > 1 ? (void*)mcml_first_pass_pl : (void*)mcml_first_pass);
     kp.gridDim = dim3(grid);
     kp.blockDim = dim3(block);
     kp.sharedMemBytes = (unsigned int)lds;
@@ -1433,7 +1520,9 @@ hipError_t mcml_launch_sums_finalize(const unsigned long long* accum, int groups
 hipError_t mcml_launch_first_pass(const FirstPassArgs& a, int grid, int block, hipStream_t st)
 {
     const size_t lds = first_pass_lds(a, block);
-    if (a.stats != nullptr) {
+    if (a.A_rz != nullptr) {
+        hipLaunchKernelGGL(mcml_first_pass_grids, dim3(grid), dim3(block), lds, st, a);
+    } else if (a.stats != nullptr) {
         if (a.row_stride > 1) hipLaunchKernelGGL(mcml_first_pass_pl_stats, dim3(grid), dim3(block), lds, st, a);
         else hipLaunchKernelGGL(mcml_first_pass_stats, dim3(grid), dim3(block), lds, st, a);
     } else if (a.row_stride > 1) {
@@ -1471,6 +1560,8 @@ hipError_t mcml_launch_groups(const uint64_t* keys, uint64_t n, int nsets, int n
 hipError_t mcml_occupancy_first_pass(const FirstPassArgs& a, int block, int* blocks_per_cu)
 {
     const size_t lds = first_pass_lds(a, block);
+    if (a.A_rz != nullptr)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_first_pass_grids, block, lds);
     if (a.row_stride > 1)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_first_pass_pl, block, lds);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_first_pass, block, lds);

@@ -83,16 +83,19 @@ REPLAY_CASES = ["fiber_busy", "ring_busy", "HW8_A", "head_model", "hemisphere_g0
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("absorbance", [True, False], ids=["replay_kernel", "inline_pathlength"])
+@pytest.mark.parametrize("mode", ["grids_in_pass", "replay_pass", "inline_pathlength"])
 @pytest.mark.parametrize("name", REPLAY_CASES)
-def test_replay_matches_oracle(ctx, name, absorbance):
-    """With -A the library replays detected photons (absorbance maps need it); without -A
-    the first pass records the pathlengths itself.  Both must equal the oracle's replay."""
+def test_replay_matches_oracle(ctx, name, mode):
+    """With -A the detected photons are replayed into the absorbance maps, by their own lanes
+    within the first pass or in a replay pass of their own; without -A the first pass
+    records the pathlengths itself.  All must equal the oracle's replay."""
+    absorbance = mode != "inline_pathlength"
     sim = CASES[name]()
     sim.number_photons = min(sim.number_photons, 8000)
     sim.grid_nr, sim.grid_nz, sim.grid_dr, sim.grid_dz = 60, 50, 0.02, 0.01
-    print("case", name, "photons", sim.number_photons, "absorbance", absorbance, flush=True)
-    res = ctx.run(sim, make_opts(replay=True, pathlength=True, absorbance=absorbance), seed=SEED)
+    print("case", name, "photons", sim.number_photons, "mode", mode, flush=True)
+    res = ctx.run(sim, make_opts(replay=True, pathlength=True, absorbance=absorbance,
+                                 separate_replay=mode == "replay_pass"), seed=SEED)
     d = O.derive(sim)
     L, S = sim.number_layers, sim.num_sds
     nz, nr = 50, 60
@@ -242,6 +245,10 @@ def test_batch_absorbance_grids_match_oracle(ctx):
     for s in sims:
         s.grid_nr, s.grid_nz, s.grid_dr, s.grid_dz = 40, 30, 0.01, 0.004
     res = ctx.run_batch(sims, make_opts(replay=True, pathlength=True, absorbance=True), seed=SEED)
+    sep = ctx.run_batch(sims, make_opts(replay=True, pathlength=True, absorbance=True, separate_replay=True),
+                        seed=SEED)
+    assert np.array_equal(sep.A_rz, res.A_rz) and np.array_equal(sep.A0_z, res.A0_z)
+    assert np.array_equal(_bits(sep.pathlength), _bits(res.pathlength))
     for j, sim in enumerate(sims):
         d = O.derive(sim)
         for s in range(1, sim.num_sds + 1):
@@ -342,3 +349,21 @@ def test_drain_compaction_does_not_change_results(ctx, name):
     assert off.steps == int(cnt[0])
     got = photon_order(on.records)
     assert np.array_equal(got["key"], rec["key"]) and _same_floats(got["weight"], rec["weight"])
+
+
+@pytest.mark.gpu
+def test_absorbance_with_many_detections_takes_the_replay_pass(ctx):
+    """More detections than the pass has lanes: the first pass records the rows and the replay
+    pass books the grids.  Rows and grids equal those of the separate replay path (which
+    the smaller cases pin to the oracle), and the oracle's replays of the first photons'
+    detections land in the same cells."""
+    sim = F.hemisphere(1_500_000, mua=1.0, mus=100.0, g=0.9, n=1.4)
+    sim.grid_nr, sim.grid_nz, sim.grid_dr, sim.grid_dz = 40, 30, 0.01, 0.004
+    opts = dict(replay=True, pathlength=True, absorbance=True)
+    res = ctx.run(sim, make_opts(**opts), seed=SEED)
+    print("records", len(res.records), "replay kernel ms", res.ms_replay_kernel, flush=True)
+    assert len(res.records) > 524288 and res.ms_replay_kernel > 0
+    sep = ctx.run(sim, make_opts(separate_replay=True, **opts), seed=SEED)
+    assert np.array_equal(res.A_rz, sep.A_rz) and np.array_equal(res.A0_z, sep.A0_z)
+    assert np.array_equal(_bits(res.pathlength), _bits(sep.pathlength))
+    assert np.array_equal(res.records, sep.records)
