@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--absorbance", action="store_true",
                     help="replay into the A_rz/A0_z grids (-R -A -B) on any single-set config (cfg2 always does)")
     ap.add_argument("--train-sets", type=int, default=16384, help="cfg4: simulated training sets")
-    ap.add_argument("--train-photons", type=float, default=6e4, help="cfg4: photons per training set")
+    ap.add_argument("--train-photons", type=float, default=1e6, help="cfg4: photons per training set")
     ap.add_argument("--white", type=int, default=13, help="cfg4: absorption vectors per set (White MC)")
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--heldout-sets", type=int, default=1000)
@@ -386,6 +386,8 @@ def bench_surrogate(args):
                                   args.heldout_sets, args.heldout_photons),
                    "epochs": args.epochs},
         "median_rel_err": med, "median_rel_err_per_sds": [float(v) for v in np.median(err, axis=0)],
+        "within_bound_per_sds": bool(np.all(np.median(err, axis=0) <= 0.02)),
+        "reference_rel_se_per_sds": [float(v) for v in np.median(np.sqrt(hrv), axis=0)],
         "p90_rel_err": float(np.quantile(err, 0.9)), "within_bound": bool(sur.usable),
         "mc_photons_per_s": mc_pps, "mc_photons_for_2pct": photons_2pct, "mc_evals_per_s": mc_evals_per_s,
         "gain_vs_mc": evals_per_s / mc_evals_per_s, "train_s": net.train_ms / 1e3, "data_s": t_data,
