@@ -691,6 +691,35 @@ double steady_s()
 
 } // namespace
 
+/* Consecutive runs of input lines (first, count) whose absorbance grids fit the budget; a
+ * line that alone exceeds it runs by itself. */
+static std::vector<std::pair<int, int>> plan_batches(const mcml_sim* sims, int ns, const mcml_opts& o)
+{
+    double budget = 0.0;
+    if (const char* e = getenv("MCML_BATCH_BUDGET_BYTES")) budget = strtod(e, nullptr);
+    if (budget <= 0.0) {
+        size_t free_b = 0, total_b = 0;
+        budget = hipMemGetInfo(&free_b, &total_b) == hipSuccess ? 0.5 * (double)free_b : 8e9;
+    }
+    std::vector<std::pair<int, int>> out;
+    int b0 = 0;
+    double used = 0.0;
+    for (int i = 0; i < ns; i++) {
+        const mcml_sim& m = sims[i];
+        const double grid = o.replay && o.absorbance
+                                ? 8.0 * (double)m.num_sds * (double)m.grid_nz * ((double)m.grid_nr + 1.0)
+                                : 0.0;
+        if (i > b0 && used + grid > budget) {
+            out.push_back({b0, i - b0});
+            b0 = i;
+            used = 0.0;
+        }
+        used += grid;
+    }
+    if (ns > b0) out.push_back({b0, ns - b0});
+    return out;
+}
+
 extern "C" int mcml_main(int argc, char** argv)
 {
     mcml_opts o;
@@ -781,37 +810,47 @@ extern "C" int mcml_main(int argc, char** argv)
         return 1;
     }
     const double t0 = steady_s();
-    /* all lines of input.txt in one persistent launch; outputs per simulation in order */
-    mcml_result res;
-    rc = mcml_run_batch(ctx, sims, ns, &o, seed, &res);
-    if (rc != MCML_OK) {
-        printf("Error code=%i, %s.\n", rc, mcml_last_error(ctx));
-        mcml_destroy(ctx);
-        mcml_free_sims(sims);
-        return 1;
-    }
-    uint64_t total = 0;
-    for (int i = 0; i < ns; i++) total += sims[i].number_photons;
-    for (int i = 0; i < ns; i++) {
-        printf("simulating %d\n", i);
-        printf("photon deteced number for SDSs:");
-        for (int s = 0; s < res.num_sds; s++) printf("\t%llu,", (unsigned long long)res.detected[(size_t)i * res.num_sds + s]);
-        printf("\n");
-        const double share = total ? (double)sims[i].number_photons / (double)total : 0.0;
-        mcml_summary_info info;
-        info.sim_time_s = res.ms_first_pass_wall * 1e-3 * share;
-        info.replay_time_s = res.ms_replay_wall * 1e-3 * share;
-        info.gpu_name = mcml_device_name(ctx);
-        printf("finish first run, cost %.2f secs\n", info.sim_time_s);
-        rc = mcml_write_outputs(&sims[i], &o, &res, i, argv[3], nullptr, &info);
+    /* The lines of input.txt run as few persistent launches as memory allows: consecutive
+     * lines are batched while their absorbance grids (held on the device and again in the
+     * result) fit half of the device memory that is free, or MCML_BATCH_BUDGET_BYTES when that
+     * is set.  Photon streams depend on (seed, set, photon) through seed + 2^64/phi (set + 1)
+     * only, so running lines b.. as sets 0.. under seed + 2^64/phi b gives every line the
+     * stream it has in a single batch: the outputs do not depend on the split.  Outputs are
+     * written per simulation in input order, each batch before the next runs. */
+    const std::vector<std::pair<int, int>> batches = plan_batches(sims, ns, o);
+    for (size_t bi = 0; bi < batches.size() && rc == MCML_OK; bi++) {
+        const int b0 = batches[bi].first, nb = batches[bi].second;
+        mcml_result res;
+        rc = mcml_run_batch(ctx, sims + b0, nb, &o, seed + 0x9E3779B97F4A7C15ull * (uint64_t)b0, &res);
         if (rc != MCML_OK) {
-            printf("Error code=%i writing outputs\n", rc);
+            printf("Error code=%i, %s.\n", rc, mcml_last_error(ctx));
             break;
         }
-        if (o.replay) printf("finish replay, cost %.2f secs\n", info.replay_time_s);
+        uint64_t total = 0;
+        for (int i = b0; i < b0 + nb; i++) total += sims[i].number_photons;
+        for (int i = b0; i < b0 + nb; i++) {
+            const int j = i - b0;
+            printf("simulating %d\n", i);
+            printf("photon deteced number for SDSs:");
+            for (int s = 0; s < res.num_sds; s++)
+                printf("\t%llu,", (unsigned long long)res.detected[(size_t)j * res.num_sds + s]);
+            printf("\n");
+            const double share = total ? (double)sims[i].number_photons / (double)total : 0.0;
+            mcml_summary_info info;
+            info.sim_time_s = res.ms_first_pass_wall * 1e-3 * share;
+            info.replay_time_s = res.ms_replay_wall * 1e-3 * share;
+            info.gpu_name = mcml_device_name(ctx);
+            printf("finish first run, cost %.2f secs\n", info.sim_time_s);
+            rc = mcml_write_outputs(&sims[i], &o, &res, j, argv[3], nullptr, &info);
+            if (rc != MCML_OK) {
+                printf("Error code=%i writing outputs\n", rc);
+                break;
+            }
+            if (o.replay) printf("finish replay, cost %.2f secs\n", info.replay_time_s);
+        }
+        mcml_result_free(&res);
     }
     printf("Simulation time: %.2f sec\n", steady_s() - t0);
-    mcml_result_free(&res);
     mcml_destroy(ctx);
     mcml_free_sims(sims);
     return rc == MCML_OK ? 0 : 1;

@@ -86,3 +86,29 @@ def test_cli_outputs_equal_library_results(tmp_path):
         assert np.array_equal(got, res.pathlength_records(1, s))
         amap = loaders.load_A_arr(tmp_path, s, nz=30, nr=40)
         assert amap.shape == (30, 40) and np.all(amap >= 0)
+
+
+@pytest.mark.gpu
+def test_cli_outputs_do_not_depend_on_batching(tmp_path):
+    """Lines run in batches that fit a memory budget; with MCML_BATCH_BUDGET_BYTES=1 every
+    line is a batch of its own, under a shifted seed that gives it the photon streams it has
+    in one batch: every output file (summary.json, which holds timings, aside) is the same."""
+    photons = 100000
+    outs = []
+    for budget in (None, "1"):
+        d = tmp_path / ("whole" if budget is None else "split")
+        d.mkdir()
+        js, txt = write_inputs(d, photons)
+        env = dict(os.environ)
+        if budget is not None:
+            env["MCML_BATCH_BUDGET_BYTES"] = budget
+        r = subprocess.run([M.EXE_PATH, js, txt, "output.txt", "-R", "-P", "-AP", "-A", "-S", str(SEED)],
+                           capture_output=True, text=True, cwd=d, timeout=120, env=env)
+        print("budget", budget, r.stdout[-600:], r.stderr[-600:], flush=True)
+        assert r.returncode == 0
+        outs.append(d)
+    names = sorted(f for f in os.listdir(outs[0]) if f not in ("sim_set.json", "input.txt", "summary.json"))
+    assert "output.txt" in names and any(f.startswith("A_rz_SDS_") for f in names)
+    assert names == sorted(f for f in os.listdir(outs[1]) if f not in ("sim_set.json", "input.txt", "summary.json"))
+    for f in names:
+        assert (outs[0] / f).read_bytes() == (outs[1] / f).read_bytes(), f
