@@ -370,9 +370,18 @@ __device__ __forceinline__ int survive(Photon& p)
  * end.  Drops outside the corner, or of a lane whose photon belongs to another group, go
  * straight to the global grids.  u64 sums are exact and associative, so the grids do not
  * depend on where a drop was first added (the reference's own accumulation type,
- * transport.cu:335-339; FP32 tiles would round). */
-#define TILE_Z 16
-#define TILE_R 16
+ * transport.cu:335-339; FP32 tiles would round).
+ *
+ * The tile is kept in TILE_COPIES copies, lane i adding to copy i % TILE_COPIES: the drops
+ * crowd into a few cells near the source (on the HW8_A example 7 % of the runs land in cell
+ * (0, 0) and every first drop in A0_z[0]), and lanes adding to one LDS word in the same
+ * instruction are served one after another.  With a 32 x 32 corner (all of HW8_A's runs,
+ * against 94 % for 16 x 16; the others go to contended global atomics) and two copies the
+ * HW8_A -R -A replay took 6.3 ms instead of 10.0; four copies cost more in occupancy than
+ * they saved, u32 cells were slower, and without any atomics it takes 2.4 ms. */
+#define TILE_Z 32
+#define TILE_R 32
+#define TILE_COPIES 2
 #define TILE_WORDS (TILE_Z * TILE_R + TILE_Z)
 
 struct GridLane {
@@ -1103,20 +1112,25 @@ __device__ __forceinline__ uint64_t take_jobs(JobQueue& q, uint64_t need, bool m
     return id;
 }
 
-/* Add a wavefront's tile into its group's global grids and clear it (wave-cooperative). */
+/* Add a wavefront's tile (all copies; `tile` is the calling lane's copy) into its group's global
+ * grids and clear it (wave-cooperative). */
 __device__ __forceinline__ void flush_tile(unsigned long long* tile, uint32_t grp, const ReplayArgs& a)
 {
     if (grp == ~0u) return;
     unsigned long long* rz = a.A_rz + (uint64_t)grp * (uint64_t)a.nz * (uint64_t)a.nr;
     unsigned long long* z0 = a.A0_z + (uint64_t)grp * (uint64_t)a.nz;
+    unsigned long long* base = tile - (lane_id() % TILE_COPIES) * TILE_WORDS; /* copy 0 */
     for (int i = lane_id(); i < TILE_WORDS; i += 64) {
-        const unsigned long long v = tile[i];
+        unsigned long long v = 0ull;
+        for (int c = 0; c < TILE_COPIES; c++) {
+            v += base[c * TILE_WORDS + i];
+            base[c * TILE_WORDS + i] = 0ull;
+        }
         if (v == 0ull) continue;
         if (i < TILE_Z * TILE_R)
             atomicAdd(&rz[(uint64_t)(i / TILE_R) * (uint64_t)a.nr + (uint64_t)(i % TILE_R)], v);
         else
             atomicAdd(&z0[i - TILE_Z * TILE_R], v);
-        tile[i] = 0ull;
     }
 }
 
@@ -1126,7 +1140,7 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
     stage_tables();
     /* per-lane pathlength accumulators, layer-major so lanes never share a bank word */
     __shared__ float pl_lds[MCML_MAX_LAYERS * REPLAY_BLOCK];
-    __shared__ unsigned long long grid_tiles[REPLAY_BLOCK / 64][TILE_WORDS];
+    __shared__ unsigned long long grid_tiles[REPLAY_BLOCK / 64][TILE_WORDS * TILE_COPIES];
     float* pl_lane = pl_lds + threadIdx.x;
     const int L = a.num_layers;
     const int bnd_min = a.bnd_min < 1 ? 1 : a.bnd_min;
@@ -1151,8 +1165,8 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
         jq.g = jq.g_start = lo;
     }
     if (a.A_rz != nullptr) {
-        gl.tile = grid_tiles[threadIdx.x >> 6];
-        for (int i = lane_id(); i < TILE_WORDS; i += 64) gl.tile[i] = 0ull;
+        gl.tile = grid_tiles[threadIdx.x >> 6] + (lane_id() % TILE_COPIES) * TILE_WORDS;
+        for (int i = lane_id(); i < TILE_WORDS * TILE_COPIES; i += 64) grid_tiles[threadIdx.x >> 6][i] = 0ull;
     }
     uint32_t n_int = 0;
 
