@@ -156,6 +156,12 @@ struct ReplayArgs {
     int32_t num_sds, nr, nz, pad;
     float rdr, rdz;            /* 1/dr, 1/dz: bins are trunc(r * rdr), trunc(z * rdz) */
     unsigned long long* missed; /* jobs whose replay ended without the detection (never expected) */
+    /* staging copies of the grids' tile corner (NULL = none): [stage_k][ngroups][tile words],
+     * zero at launch; a drop in the corner that the wavefront's LDS tile cannot take goes to
+     * copy (wavefront index mod stage_k) of its group, and mcml_replay_stage_reduce adds the
+     * copies into the grids after the replay */
+    unsigned long long* stage;
+    int32_t stage_k, stage_pad;
 };
 
 /* Reference launch shape (transport.cu:57-147, 341-471): one photon per thread, at most

@@ -36,6 +36,7 @@ hipError_t mcml_occupancy_first_pass_sums(const FirstPassArgs& a, int block, int
 hipError_t mcml_launch_clear(unsigned long long* p, uint64_t n, hipStream_t st);
 hipError_t mcml_launch_sums_finalize(const unsigned long long* accum, int groups, double* reflectance,
                                      uint64_t* detected, hipStream_t st);
+/* the replay, then (when a.stage is set) the reduction of its staging copies into the grids */
 hipError_t mcml_launch_replay(const ReplayArgs& a, int grid, hipStream_t st);
 hipError_t mcml_launch_dump(const GatherArgs& a, int grid, hipStream_t st);
 hipError_t mcml_launch_groups(const uint64_t* keys, uint64_t n, int nsets, int nsds, uint64_t* offsets,
@@ -45,6 +46,7 @@ hipError_t mcml_sort_keys(void* temp, size_t& temp_bytes, const uint64_t* keys_i
                           uint32_t* idx_out, uint64_t n, hipStream_t st);
 hipError_t mcml_occupancy_replay(int* blocks_per_cu);
 int mcml_replay_block();
+size_t mcml_replay_stage_words(); /* words of one group's tile corner (A_rz corner + A0_z rows) */
 /* reference launch shape: bytes of one slot's saved state; init and one step-budget launch */
 size_t mcml_refshape_slot_bytes();
 hipError_t mcml_launch_refshape_init(const RefShapeArgs& a, int block, hipStream_t st);

@@ -53,6 +53,9 @@ static_assert(kCtrlFirstPass + 2 <= kCtrlWords, "two words for the math self-tes
 constexpr int kDefaultSpillBelow = 32; /* drain: a wavefront with at most this many photons left spills them */
 constexpr uint64_t kFinalDrain = 16384; /* a drain launch of at most this many photons runs them to the end */
 constexpr size_t kStageBytes = 128u << 20; /* pinned staging buffer for large read-backs */
+/* replay: at most this many staging copies of the grids' corner, in at most this many bytes */
+constexpr size_t kReplayStageCopies = 64;
+constexpr size_t kReplayStageBytes = 64u << 20;
 
 struct DevBuf {
     void* p = nullptr;
@@ -82,6 +85,7 @@ struct mcml_ctx {
     DevBuf offsets, sums;             /* per (set, SDS) group: start offsets, FP64 sums */
     DevBuf rrows, arz, a0z;           /* replay rows and absorbance grids */
     DevBuf rgrp, rgq;                 /* replay: job range and hand-out counter per (set, SDS) */
+    DevBuf rstage;                    /* replay: staging copies of the grids' tile corner */
     DevBuf wrows, woff, wold, wnew, wout; /* White Monte Carlo inputs and output */
     DevBuf ref_state, ref_active, ref_fibers, ref_spill_key, ref_spill_w; /* reference launch shape */
     DevBuf spill[2];                  /* drain compaction: photons handed from one launch to the next */
@@ -521,7 +525,7 @@ extern "C" void mcml_destroy(mcml_ctx* c)
                       &c->rows,  &c->weights,   &c->precs,   &c->offsets, &c->sums,  &c->rrows, &c->arz,
                       &c->a0z,   &c->wrows,     &c->woff,    &c->wold,    &c->wnew,  &c->wout,
                       &c->ref_state, &c->ref_active, &c->ref_fibers, &c->ref_spill_key, &c->ref_spill_w,
-                      &c->rgrp,      &c->rgq,        &c->spill[0],   &c->spill[1]};
+                      &c->rgrp,      &c->rgq,        &c->spill[0],   &c->spill[1], &c->rstage};
     for (DevBuf* b : bufs) release(*b);
     mcml_graph_release(c->graph);
     for (int i = 0; i < 2; i++) {
@@ -1116,6 +1120,22 @@ int replay_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_op
     a.rdr = gs.rdr;
     a.rdz = gs.rdz;
     a.missed = c->ctrl + kCtrlMissed;
+    if (gs.A_rz != nullptr) {
+        /* A drop in the grids' corner whose group is not the one its wavefront's LDS tile holds
+         * (a wavefront moving between groups holds photons of two) would be a global atomic on
+         * the grids' hottest cells, which every wavefront hits: serialised there, they took half
+         * of the HW8_A replay.  Such drops go to one of stage_k copies of the corner instead,
+         * chosen by wavefront, and a small kernel adds the copies in after the replay. */
+        const size_t per_copy = groups * mcml_replay_stage_words() * sizeof(unsigned long long);
+        const size_t k = std::min<size_t>(kReplayStageCopies, kReplayStageBytes / std::max<size_t>(per_copy, 1));
+        if (k >= 2) {
+            int rc;
+            if ((rc = ensure(c, c->rstage, k * per_copy))) return rc;
+            HIP_TRY(c, mcml_launch_clear(as<unsigned long long>(c->rstage), k * per_copy / 8, c->stream));
+            a.stage = as<unsigned long long>(c->rstage);
+            a.stage_k = (int32_t)k;
+        }
+    }
     HIP_TRY(c, mcml_launch_reset(c->ctrl + kCtrlMissed, 1, c->stream));
     int occ = 0;
     HIP_TRY(c, mcml_occupancy_replay(&occ));

@@ -367,10 +367,13 @@ __device__ __forceinline__ int survive(Photon& p)
  * Each wavefront owns an LDS tile of the hot corner of one (set, SDS) group's grids (z bins
  * < TILE_Z, and r bins < TILE_R for A_rz), accumulated with u64 LDS atomics and flushed to
  * the global grids with u64 atomics when the wavefront moves on to another group and at the
- * end.  Drops outside the corner, or of a lane whose photon belongs to another group, go
- * straight to the global grids.  u64 sums are exact and associative, so the grids do not
- * depend on where a drop was first added (the reference's own accumulation type,
- * transport.cu:335-339; FP32 tiles would round).
+ * end.  A corner drop of a lane whose photon belongs to another group than the tile's (a
+ * wavefront moving between groups holds photons of both for a while) goes to the wavefront's
+ * staging copy of the corner in global memory (ReplayArgs.stage, one of up to 64 copies, so
+ * few wavefronts share a word), added into the grids by mcml_replay_stage_reduce after the
+ * replay; drops outside the corner go straight to the global grids.  u64 sums are exact and
+ * associative, so the grids do not depend on where a drop was first added (the reference's
+ * own accumulation type, transport.cu:335-339; FP32 tiles would round).
  *
  * The tile is kept in TILE_COPIES copies, lane i adding to copy i % TILE_COPIES: the drops
  * crowd into a few cells near the source (on the HW8_A example 7 % of the runs land in cell
@@ -378,7 +381,9 @@ __device__ __forceinline__ int survive(Photon& p)
  * instruction are served one after another.  With a 32 x 32 corner (all of HW8_A's runs,
  * against 94 % for 16 x 16; the others go to contended global atomics) and two copies the
  * HW8_A -R -A replay took 6.3 ms instead of 10.0; four copies cost more in occupancy than
- * they saved, u32 cells were slower, and without any atomics it takes 2.4 ms. */
+ * they saved, u32 cells were slower.  Half of those 6.3 ms were the other-group drops, as
+ * global atomics on the grids' hottest cells that every wavefront hits: with the staging
+ * copies the replay takes 3.5 ms (without any grid atomics, 2.4 ms). */
 #define TILE_Z 32
 #define TILE_R 32
 #define TILE_COPIES 2
@@ -388,6 +393,7 @@ struct GridLane {
     unsigned long long* a_rz;   /* this lane's (set, SDS) slabs */
     unsigned long long* a0_z;
     unsigned long long* tile;   /* the wavefront's LDS tile (NULL without grids) */
+    unsigned long long* stage;  /* this lane's group in the wavefront's staging copy (or NULL) */
     uint64_t rl_key;            /* run in progress: z bin << 32 | r bin */
     uint64_t rl_w;
     uint32_t grp;               /* this lane's (set, SDS) group: set * S + sds - 1 */
@@ -398,8 +404,11 @@ __device__ __forceinline__ void flush_run(GridLane& gl, int nr)
 {
     if (gl.rl_w == 0) return;
     const uint32_t zb = (uint32_t)(gl.rl_key >> 32), rb = (uint32_t)gl.rl_key;
-    if (gl.grp == gl.tile_grp && zb < TILE_Z && rb < TILE_R)
+    const bool corner = zb < TILE_Z && rb < TILE_R;
+    if (corner && gl.grp == gl.tile_grp)
         atomicAdd(&gl.tile[zb * TILE_R + rb], (unsigned long long)gl.rl_w);
+    else if (corner && gl.stage != nullptr)
+        atomicAdd(&gl.stage[zb * TILE_R + rb], (unsigned long long)gl.rl_w);
     else
         atomicAdd(&gl.a_rz[(uint64_t)zb * (uint64_t)nr + rb], (unsigned long long)gl.rl_w);
     gl.rl_w = 0;
@@ -419,8 +428,10 @@ __device__ __forceinline__ int interact(Photon& p, float s, GridLane& gl, const 
             qz = fminf(fmaxf(qz, 0.0f), (float)(ra->nz - 1));
             const int zb = (int)qz;
             if (p.first) {
-                if (gl.grp == gl.tile_grp && zb < TILE_Z)
+                if (zb < TILE_Z && gl.grp == gl.tile_grp)
                     atomicAdd(&gl.tile[TILE_Z * TILE_R + zb], (unsigned long long)w_temp);
+                else if (zb < TILE_Z && gl.stage != nullptr)
+                    atomicAdd(&gl.stage[TILE_Z * TILE_R + zb], (unsigned long long)w_temp);
                 else
                     atomicAdd(&gl.a0_z[zb], (unsigned long long)w_temp);
             } else {
@@ -1150,7 +1161,12 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
     uint64_t job = 0;
     uint32_t jset = 0, jsds = 1;
     const DevSet* S = a.sets;
-    GridLane gl = {nullptr, nullptr, nullptr, ~0ull, 0, 0, ~0u};
+    GridLane gl = {nullptr, nullptr, nullptr, nullptr, ~0ull, 0, 0, ~0u};
+    /* this wavefront's staging copy: wavefront index mod stage_k */
+    unsigned long long* stage_copy =
+        a.stage == nullptr ? nullptr
+                           : a.stage + (((uint64_t)blockIdx.x * (REPLAY_BLOCK / 64) + (threadIdx.x >> 6)) %
+                                        (uint64_t)a.stage_k) * (uint64_t)a.ngroups * TILE_WORDS;
     JobQueue jq = {0, 0, 0, 0, a.ngroups < 1};
     if (!jq.dry) { /* start in the group holding this wavefront's share of the job order */
         const uint64_t w = ((uint64_t)blockIdx.x * REPLAY_BLOCK + threadIdx.x) >> 6;
@@ -1188,6 +1204,7 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
                     gl.grp = jset * (uint32_t)a.num_sds + (jsds - 1);
                     gl.a_rz = a.A_rz + (uint64_t)gl.grp * (uint64_t)a.nz * (uint64_t)a.nr;
                     gl.a0_z = a.A0_z + (uint64_t)gl.grp * (uint64_t)a.nz;
+                    gl.stage = stage_copy == nullptr ? nullptr : stage_copy + (uint64_t)gl.grp * TILE_WORDS;
                 }
                 alive = true;
             }
@@ -1240,6 +1257,24 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
         }
     }
     if (gl.tile != nullptr) flush_tile(gl.tile, gl.tile_grp, a);
+}
+
+/* Add the replay's staging copies into the grids: one thread per (group, corner word) sums the
+ * stage_k copies in a fixed order.  It runs after mcml_replay on the same stream, alone on its
+ * cells, so a plain add suffices; the sums are exact either way. */
+extern "C" __global__ void __launch_bounds__(256) mcml_replay_stage_reduce(ReplayArgs a)
+{
+    const uint64_t n = (uint64_t)a.ngroups * TILE_WORDS;
+    for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (uint64_t)gridDim.x * 256) {
+        unsigned long long v = 0ull;
+        for (int k = 0; k < a.stage_k; k++) v += a.stage[(uint64_t)k * n + t];
+        if (v == 0ull) continue;
+        const uint64_t g = t / TILE_WORDS, i = t % TILE_WORDS;
+        if (i < TILE_Z * TILE_R)
+            a.A_rz[(g * (uint64_t)a.nz + i / TILE_R) * (uint64_t)a.nr + i % TILE_R] += v;
+        else
+            a.A0_z[g * (uint64_t)a.nz + (i - TILE_Z * TILE_R)] += v;
+    }
 }
 
 /* ----------------------------------------------------------------- sorted records */
@@ -1550,8 +1585,15 @@ hipError_t mcml_launch_first_pass(const FirstPassArgs& a, int grid, int block, h
 hipError_t mcml_launch_replay(const ReplayArgs& a, int grid, hipStream_t st)
 {
     hipLaunchKernelGGL(mcml_replay, dim3(grid), dim3(REPLAY_BLOCK), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || a.stage == nullptr) return e;
+    const uint64_t n = (uint64_t)a.ngroups * TILE_WORDS;
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(mcml_replay_stage_reduce, dim3((unsigned)blocks), dim3(256), 0, st, a);
     return hipGetLastError();
 }
+
+size_t mcml_replay_stage_words() { return TILE_WORDS; }
 
 hipError_t mcml_launch_dump(const GatherArgs& a, int grid, hipStream_t st)
 {
