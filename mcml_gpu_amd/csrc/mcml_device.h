@@ -118,10 +118,11 @@ struct FirstPassArgs {
 
 /* Fields of one spilled photon (each a 32-bit column of the SoA spill arrays): position,
  * direction, weight, layer, steps, first-scatter flag, the RNG state (3 words), the running
- * pathlength and its layer, the record key (2 words), the pending boundary, then the
- * per-layer pathlength totals. */
+ * pathlength and its layer, the record key (2 words), the pending boundary, the replay state
+ * of mcml_first_pass_grids (detectors still to replay for, the drop run's cell and weight;
+ * no replay in the other variants), then the per-layer pathlength totals. */
 enum { SP_X = 0, SP_Y, SP_Z, SP_DX, SP_DY, SP_DZ, SP_W, SP_LAYER, SP_STEPS, SP_FIRST, SP_RNG_LO, SP_RNG_HI, SP_RNG_A,
-       SP_PL_CUR, SP_PL_LAYER, SP_KEY_LO, SP_KEY_HI, SP_PENDING, SP_PL0 };
+       SP_PL_CUR, SP_PL_LAYER, SP_KEY_LO, SP_KEY_HI, SP_PENDING, SP_RP_MASK, SP_RL_KEY, SP_RL_W, SP_PL0 };
 #define SPILL_FIELDS(L) (SP_PL0 + (L))
 
 /* Per-(set, SDS) accumulators of the sums sink.  A group's detected weight is summed exactly

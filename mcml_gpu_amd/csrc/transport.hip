@@ -740,6 +740,11 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                 const uint32_t pb = __float_as_uint(col[SP_PENDING * cs]);
                 pending = (pb & 1u) != 0u;
                 pend_layer = (int)(pb >> 1);
+                if (GRIDS) { /* a replay in progress continues where it was spilled */
+                    rp_mask = __float_as_uint(col[SP_RP_MASK * cs]);
+                    rl_key = __float_as_uint(col[SP_RL_KEY * cs]);
+                    rl_w = __float_as_uint(col[SP_RL_W * cs]);
+                }
                 if (PL)
                     for (int l = 0; l < L; l++) pl_lane[l * pl_stride] = col[(SP_PL0 + l) * cs];
                 set = (uint32_t)(key >> MCML_KEY_SET_SHIFT);
@@ -785,6 +790,11 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                     col[SP_KEY_LO * cs] = __uint_as_float((uint32_t)key);
                     col[SP_KEY_HI * cs] = __uint_as_float((uint32_t)(key >> 32));
                     col[SP_PENDING * cs] = __uint_as_float((pending ? 1u : 0u) | ((uint32_t)pend_layer << 1));
+                    /* written by every variant (no replay: mask 0), as a drain launch of
+                     * mcml_first_pass_grids may continue what the main launch spilled */
+                    col[SP_RP_MASK * cs] = __uint_as_float(rp_mask);
+                    col[SP_RL_KEY * cs] = __uint_as_float(rl_key);
+                    col[SP_RL_W * cs] = __uint_as_float(rl_w);
                     if (PL)
                         for (int l = 0; l < L; l++) col[(SP_PL0 + l) * cs] = pl_lane[l * pl_stride];
                     c_int += n_int; /* the photon's interactions so far; its steps count where it ends */
