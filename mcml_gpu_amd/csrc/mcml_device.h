@@ -114,6 +114,11 @@ struct FirstPassArgs {
     unsigned long long* replay_missed; /* replays that ended without the detection (never expected) */
     int32_t nr, nz;
     float rdr, rdz;                    /* 1/dr, 1/dz: bins are trunc(r * rdr), trunc(z * rdz) */
+    /* staging copies of the grids' corner for the replays' drops (NULL = none), laid out as
+     * ReplayArgs.stage: [stage_k][stage_groups][corner words], copy = wavefront index mod
+     * stage_k, added into the grids by mcml_launch_stage_reduce after the last drain launch */
+    unsigned long long* stage;
+    int32_t stage_k, stage_groups;
 };
 
 /* Fields of one spilled photon (each a 32-bit column of the SoA spill arrays): position,
@@ -159,7 +164,7 @@ struct ReplayArgs {
     unsigned long long* missed; /* jobs whose replay ended without the detection (never expected) */
     /* staging copies of the grids' tile corner (NULL = none): [stage_k][ngroups][tile words],
      * zero at launch; a drop in the corner that the wavefront's LDS tile cannot take goes to
-     * copy (wavefront index mod stage_k) of its group, and mcml_replay_stage_reduce adds the
+     * copy (wavefront index mod stage_k) of its group, and mcml_stage_reduce adds the
      * copies into the grids after the replay */
     unsigned long long* stage;
     int32_t stage_k, stage_pad;

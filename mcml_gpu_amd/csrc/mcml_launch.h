@@ -38,6 +38,9 @@ hipError_t mcml_launch_sums_finalize(const unsigned long long* accum, int groups
                                      uint64_t* detected, hipStream_t st);
 /* the replay, then (when a.stage is set) the reduction of its staging copies into the grids */
 hipError_t mcml_launch_replay(const ReplayArgs& a, int grid, hipStream_t st);
+/* add k staging copies [k][ngroups][mcml_replay_stage_words()] into the grids */
+hipError_t mcml_launch_stage_reduce(const unsigned long long* stage, int k, int ngroups, unsigned long long* A_rz,
+                                    unsigned long long* A0_z, int nz, int nr, hipStream_t st);
 hipError_t mcml_launch_dump(const GatherArgs& a, int grid, hipStream_t st);
 hipError_t mcml_launch_groups(const uint64_t* keys, uint64_t n, int nsets, int nsds, uint64_t* offsets,
                               const float* w, double* sums, hipStream_t st);

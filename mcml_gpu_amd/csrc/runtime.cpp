@@ -604,7 +604,26 @@ int validate_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, std::vector<mcm
     return MCML_OK;
 }
 
-/* First pass: leaves n_rec records in c->rec_key / c->rec_row (row_stride floats each). */
+/* Staging copies of the grids' corner for `groups` (set, SDS) groups (ReplayArgs.stage): as
+ * many copies as fit kReplayStageCopies and kReplayStageBytes, cleared by a kernel.  A drop
+ * in the corner whose wavefront cannot add it to an LDS tile goes to its wavefront's copy
+ * rather than to the grids, whose few hottest cells every replay hits: atomics serialised
+ * there (half of the HW8_A replay).  *k = 0 (no copies) when fewer than two fit. */
+int stage_copies(mcml_ctx* c, size_t groups, unsigned long long** stage, int32_t* k)
+{
+    *stage = nullptr;
+    *k = 0;
+    const size_t per_copy = groups * mcml_replay_stage_words() * sizeof(unsigned long long);
+    const size_t n = std::min<size_t>(kReplayStageCopies, kReplayStageBytes / std::max<size_t>(per_copy, 1));
+    if (n < 2) return MCML_OK;
+    int rc;
+    if ((rc = ensure(c, c->rstage, n * per_copy))) return rc;
+    HIP_TRY(c, mcml_launch_clear(as<unsigned long long>(c->rstage), n * per_copy / 8, c->stream));
+    *stage = as<unsigned long long>(c->rstage);
+    *k = (int32_t)n;
+    return MCML_OK;
+}
+
 /* Absorbance grids the first pass books its replays into (-R -A). */
 struct GridSpec {
     unsigned long long* A_rz;
@@ -613,6 +632,7 @@ struct GridSpec {
     float rdr, rdz;
 };
 
+/* First pass: leaves n_rec records in c->rec_key / c->rec_row (row_stride floats each). */
 int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opts& o, uint64_t seed, int row_stride,
                mcml_result* out, uint64_t& n_rec, const GridSpec* grids = nullptr, bool* grids_done = nullptr)
 {
@@ -775,6 +795,8 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
         d.rdr = grids->rdr;
         d.rdz = grids->rdz;
         d.replay_missed = c->ctrl + kCtrlMissed;
+        d.stage_groups = (int32_t)(der.size() * (size_t)der[0].num_sds);
+        if ((rc = stage_copies(c, (size_t)d.stage_groups, &d.stage, &d.stage_k))) return rc;
     }
     uint64_t n_jobs = grid_drains ? n_rec : 0; /* the main launch's detections, replayed by the first drain */
     while (n_sp + n_jobs > 0) {
@@ -809,6 +831,8 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
         n_jobs = 0;
         from ^= 1;
     }
+    if (grid_drains && d.stage != nullptr)
+        HIP_TRY(c, mcml_launch_stage_reduce(d.stage, d.stage_k, d.stage_groups, d.A_rz, d.A0_z, d.nz, d.nr, c->stream));
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     HIP_TRY(c, hipEventSynchronize(c->ev1));
     const double ms_kernel = ms_between(c->ev0, c->ev1);
@@ -1115,21 +1139,9 @@ int replay_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_op
     a.rdr = gs.rdr;
     a.rdz = gs.rdz;
     a.missed = c->ctrl + kCtrlMissed;
-    if (gs.A_rz != nullptr) {
-        /* A drop in the grids' corner whose group is not the one its wavefront's LDS tile holds
-         * (a wavefront moving between groups holds photons of two) would be a global atomic on
-         * the grids' hottest cells, which every wavefront hits: serialised there, they took half
-         * of the HW8_A replay.  Such drops go to one of stage_k copies of the corner instead,
-         * chosen by wavefront, and a small kernel adds the copies in after the replay. */
-        const size_t per_copy = groups * mcml_replay_stage_words() * sizeof(unsigned long long);
-        const size_t k = std::min<size_t>(kReplayStageCopies, kReplayStageBytes / std::max<size_t>(per_copy, 1));
-        if (k >= 2) {
-            int rc;
-            if ((rc = ensure(c, c->rstage, k * per_copy))) return rc;
-            HIP_TRY(c, mcml_launch_clear(as<unsigned long long>(c->rstage), k * per_copy / 8, c->stream));
-            a.stage = as<unsigned long long>(c->rstage);
-            a.stage_k = (int32_t)k;
-        }
+    if (gs.A_rz != nullptr) { /* corner drops of a group other than the wavefront's tile's */
+        const int rc = stage_copies(c, groups, &a.stage, &a.stage_k);
+        if (rc) return rc;
     }
     HIP_TRY(c, mcml_launch_reset(c->ctrl + kCtrlMissed, 1, c->stream));
     int occ = 0;

@@ -370,7 +370,7 @@ __device__ __forceinline__ int survive(Photon& p)
  * end.  A corner drop of a lane whose photon belongs to another group than the tile's (a
  * wavefront moving between groups holds photons of both for a while) goes to the wavefront's
  * staging copy of the corner in global memory (ReplayArgs.stage, one of up to 64 copies, so
- * few wavefronts share a word), added into the grids by mcml_replay_stage_reduce after the
+ * few wavefronts share a word), added into the grids by mcml_stage_reduce after the
  * replay; drops outside the corner go straight to the global grids.  u64 sums are exact and
  * associative, so the grids do not depend on where a drop was first added (the reference's
  * own accumulation type, transport.cu:335-339; FP32 tiles would round).
@@ -622,11 +622,25 @@ __device__ __forceinline__ void acc_weight(unsigned long long* e, float w)
  * ones to A_rz, merged while consecutive drops stay in one cell.  The run is kept in two
  * registers: its cell as z bin << 16 | r bin (grids of fewer than 65536 bins a side; ~0 =
  * none) and its u32 weight, flushed before it would wrap.  Cells are addressed by 32-bit
- * byte offsets from the grids' base (grids below 4 GiB), one u64 atomic per run. */
+ * byte offsets from the grids' base (grids below 4 GiB), one u64 atomic per run.  A drop in
+ * the grids' corner goes to the wavefront's staging copy instead when there is one: the
+ * corner's few hottest cells take the drops of every replay, and atomics serialised on them
+ * (as in mcml_replay). */
+__device__ __forceinline__ unsigned long long* drop_stage(const FirstPassArgs& a, uint32_t grp)
+{
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    return a.stage + ((uint64_t)(wave % (uint32_t)a.stage_k) * (uint64_t)a.stage_groups + grp) * TILE_WORDS;
+}
+
 __device__ __forceinline__ void flush_drop_run(const FirstPassArgs& a, uint32_t grp, uint32_t rl_key, uint32_t rl_w)
 {
     if (rl_w == 0) return;
-    const uint32_t cell = (grp * (uint32_t)a.nz + (rl_key >> 16)) * (uint32_t)a.nr + (rl_key & 0xFFFFu);
+    const uint32_t zb = rl_key >> 16, rb = rl_key & 0xFFFFu;
+    if (a.stage != nullptr && zb < TILE_Z && rb < TILE_R) {
+        atomicAdd(drop_stage(a, grp) + zb * TILE_R + rb, (unsigned long long)rl_w);
+        return;
+    }
+    const uint32_t cell = (grp * (uint32_t)a.nz + zb) * (uint32_t)a.nr + rb;
     atomicAdd(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(a.A_rz) + cell * 8u),
               (unsigned long long)rl_w);
 }
@@ -639,6 +653,10 @@ __device__ __forceinline__ void book_drop(const FirstPassArgs& a, const Photon& 
     const uint32_t zb = (uint32_t)(int)qz;
     const uint32_t wd = (uint32_t)w_temp; /* a drop is at most the weight, below 2^32 */
     if (p.first) {
+        if (a.stage != nullptr && zb < TILE_Z) {
+            atomicAdd(drop_stage(a, grp) + TILE_Z * TILE_R + zb, (unsigned long long)wd);
+            return;
+        }
         const uint32_t cell = grp * (uint32_t)a.nz + zb;
         atomicAdd(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(a.A0_z) + cell * 8u),
                   (unsigned long long)wd);
@@ -1269,21 +1287,24 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
     if (gl.tile != nullptr) flush_tile(gl.tile, gl.tile_grp, a);
 }
 
-/* Add the replay's staging copies into the grids: one thread per (group, corner word) sums the
- * stage_k copies in a fixed order.  It runs after mcml_replay on the same stream, alone on its
- * cells, so a plain add suffices; the sums are exact either way. */
-extern "C" __global__ void __launch_bounds__(256) mcml_replay_stage_reduce(ReplayArgs a)
+/* Add staging copies of the grids' corner ([k][ngroups][TILE_WORDS]) into the grids: one
+ * thread per (group, corner word) sums the k copies in a fixed order.  It runs after the
+ * kernels that fill them, on the same stream and alone on its cells, so a plain add
+ * suffices; the sums are exact either way. */
+extern "C" __global__ void __launch_bounds__(256) mcml_stage_reduce(const unsigned long long* stage, int k, int ngroups,
+                                                                    unsigned long long* A_rz, unsigned long long* A0_z,
+                                                                    int nz, int nr)
 {
-    const uint64_t n = (uint64_t)a.ngroups * TILE_WORDS;
+    const uint64_t n = (uint64_t)ngroups * TILE_WORDS;
     for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (uint64_t)gridDim.x * 256) {
         unsigned long long v = 0ull;
-        for (int k = 0; k < a.stage_k; k++) v += a.stage[(uint64_t)k * n + t];
+        for (int c = 0; c < k; c++) v += stage[(uint64_t)c * n + t];
         if (v == 0ull) continue;
         const uint64_t g = t / TILE_WORDS, i = t % TILE_WORDS;
         if (i < TILE_Z * TILE_R)
-            a.A_rz[(g * (uint64_t)a.nz + i / TILE_R) * (uint64_t)a.nr + i % TILE_R] += v;
+            A_rz[(g * (uint64_t)nz + i / TILE_R) * (uint64_t)nr + i % TILE_R] += v;
         else
-            a.A0_z[g * (uint64_t)a.nz + (i - TILE_Z * TILE_R)] += v;
+            A0_z[g * (uint64_t)nz + (i - TILE_Z * TILE_R)] += v;
     }
 }
 
@@ -1597,9 +1618,16 @@ hipError_t mcml_launch_replay(const ReplayArgs& a, int grid, hipStream_t st)
     hipLaunchKernelGGL(mcml_replay, dim3(grid), dim3(REPLAY_BLOCK), 0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || a.stage == nullptr) return e;
-    const uint64_t n = (uint64_t)a.ngroups * TILE_WORDS;
+    return mcml_launch_stage_reduce(a.stage, a.stage_k, a.ngroups, a.A_rz, a.A0_z, a.nz, a.nr, st);
+}
+
+hipError_t mcml_launch_stage_reduce(const unsigned long long* stage, int k, int ngroups, unsigned long long* A_rz,
+                                    unsigned long long* A0_z, int nz, int nr, hipStream_t st)
+{
+    const uint64_t n = (uint64_t)ngroups * TILE_WORDS;
     const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(mcml_replay_stage_reduce, dim3((unsigned)blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(mcml_stage_reduce, dim3((unsigned)std::max<uint64_t>(blocks, 1)), dim3(256), 0, st, stage, k,
+                       ngroups, A_rz, A0_z, nz, nr);
     return hipGetLastError();
 }
 
