@@ -13,18 +13,21 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
 #include <ctime>
 #include <fstream>
 #include <iostream>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "../../include/mcml.h"
+#include "mcml_stream.h"
 
 namespace {
 
@@ -496,43 +499,52 @@ extern "C" int mcml_read_input(const char* sim_set_json, const char* input_txt, 
 extern "C" void mcml_free_sims(mcml_sim* sims) { free(sims); }
 
 /* ====================================================================== writers */
-extern "C" int mcml_write_outputs(const mcml_sim* sim, const mcml_opts* opts, const mcml_result* res, int set,
-                                  const char* output_txt, const char* outdir, const mcml_summary_info* info)
+/* -P: pathlength_SDS_#.bin / .txt of one (set, SDS) group (io.cpp:281-329).  Reads only
+ * res->pl_offset and the group's rows of res->pathlength. */
+static int write_pathlength_file(const mcml_opts* opts, const mcml_result* res, int set, int s, const char* outdir)
 {
-    if (!sim || !opts || !res || !output_txt || set < 0 || set >= res->nsets) return MCML_ERR_ARG;
+    const int S = res->num_sds, L = res->num_layers;
+    const size_t g = (size_t)set * (size_t)S + (size_t)s;
+    const uint64_t b = res->pl_offset[g], e = res->pl_offset[g + 1];
+    const float* recs = res->pathlength + b * (uint64_t)(L + 2);
+    if (opts->binary) {
+        std::string fn = filename(outdir, "pathlength_SDS_" + std::to_string(s + 1) + ".bin");
+        FILE* f = fopen(fn.c_str(), "wb");
+        if (!f) return MCML_ERR_IO;
+        size_t n = (size_t)(e - b) * (size_t)(L + 2);
+        if (n && fwrite(recs, sizeof(float), n, f) != n) {
+            fclose(f);
+            return MCML_ERR_IO;
+        }
+        return fclose(f) == 0 ? MCML_OK : MCML_ERR_IO;
+    }
+    TextFile f(filename(outdir, "pathlength_SDS_" + std::to_string(s + 1) + ".txt"));
+    if (!f.ok()) return MCML_ERR_IO;
+    write_rows_parallel(f, e - b, (uint64_t)(L + 2), [&](uint64_t i, std::string& out) {
+        for (int j = 0; j <= L + 1; j++) {
+            append_g(out, recs[i * (L + 2) + j]);
+            out.push_back('\t');
+        }
+        out.push_back('\n');
+    });
+    return f.close() ? MCML_OK : MCML_ERR_IO;
+}
+
+/* Every output of one simulation; `pathlength_files` = false leaves out the -P files (the
+ * executable writes those while the batch is still being read back). */
+static int write_outputs(const mcml_sim* sim, const mcml_opts* opts, const mcml_result* res, int set,
+                         const char* output_txt, const char* outdir, const mcml_summary_info* info,
+                         bool pathlength_files)
+{
     const int S = res->num_sds, L = res->num_layers;
     const double scale1 = (double)0xFFFFFFFF * (double)sim->number_photons;
     const size_t g0 = (size_t)set * (size_t)S;
     const bool replay = opts->replay && res->replayed;
-    /* -P: pathlength_SDS_#.bin / .txt (io.cpp:281-329) */
-    if (replay && opts->pathlength) {
+    if (replay && opts->pathlength && pathlength_files)
         for (int s = 0; s < S; s++) {
-            const uint64_t b = res->pl_offset[g0 + s], e = res->pl_offset[g0 + s + 1];
-            const float* recs = res->pathlength + b * (uint64_t)(L + 2);
-            if (opts->binary) {
-                std::string fn = filename(outdir, "pathlength_SDS_" + std::to_string(s + 1) + ".bin");
-                FILE* f = fopen(fn.c_str(), "wb");
-                if (!f) return MCML_ERR_IO;
-                size_t n = (size_t)(e - b) * (size_t)(L + 2);
-                if (n && fwrite(recs, sizeof(float), n, f) != n) {
-                    fclose(f);
-                    return MCML_ERR_IO;
-                }
-                fclose(f);
-            } else {
-                TextFile f(filename(outdir, "pathlength_SDS_" + std::to_string(s + 1) + ".txt"));
-                if (!f.ok()) return MCML_ERR_IO;
-                write_rows_parallel(f, e - b, (uint64_t)(L + 2), [&](uint64_t i, std::string& out) {
-                    for (int j = 0; j <= L + 1; j++) {
-                        append_g(out, recs[i * (L + 2) + j]);
-                        out.push_back('\t');
-                    }
-                    out.push_back('\n');
-                });
-                if (!f.close()) return MCML_ERR_IO;
-            }
+            const int rc = write_pathlength_file(opts, res, set, s, outdir);
+            if (rc) return rc;
         }
-    }
     /* -AP: average_pathlength.txt (io.cpp:331-343) */
     if (replay && opts->avg_pathlength) {
         std::vector<double> avg((size_t)S * L);
@@ -642,6 +654,13 @@ extern "C" int mcml_write_outputs(const mcml_sim* sim, const mcml_opts* opts, co
     return MCML_OK;
 }
 
+extern "C" int mcml_write_outputs(const mcml_sim* sim, const mcml_opts* opts, const mcml_result* res, int set,
+                                  const char* output_txt, const char* outdir, const mcml_summary_info* info)
+{
+    if (!sim || !opts || !res || !output_txt || set < 0 || set >= res->nsets) return MCML_ERR_ARG;
+    return write_outputs(sim, opts, res, set, output_txt, outdir, info, true);
+}
+
 /* ====================================================================== CLI */
 namespace {
 
@@ -690,6 +709,71 @@ double steady_s()
 }
 
 } // namespace
+
+/* The executable's -P files, written on a thread of their own while the batch is read back:
+ * group (set, SDS) is written once its rows have landed, in set-then-SDS order, which is the
+ * order of the rows and the order in which the text files are appended to. */
+struct PathlengthWriter {
+    std::mutex m;
+    std::condition_variable cv;
+    uint64_t rows = 0;      /* rows of the result on the host */
+    bool have_rows = false; /* the result's pl_offset is complete */
+    bool finished = false;  /* the batch call has returned */
+    int run_rc = MCML_OK;   /* ... with this code */
+    bool done = false;      /* the writer thread reads the result no more */
+    int rc = MCML_OK;       /* the writer's own result */
+};
+
+static void pathlength_rows_landed(void* user, const mcml_result*, uint64_t rows)
+{
+    PathlengthWriter* w = static_cast<PathlengthWriter*>(user);
+    {
+        std::lock_guard<std::mutex> lk(w->m);
+        w->rows = rows;
+        w->have_rows = true;
+    }
+    w->cv.notify_all();
+}
+
+/* The batch is about to free the result after an error: stop the writer and wait for it. */
+static void pathlength_release(void* user)
+{
+    PathlengthWriter* w = static_cast<PathlengthWriter*>(user);
+    std::unique_lock<std::mutex> lk(w->m);
+    w->finished = true;
+    w->run_rc = MCML_ERR_DEVICE;
+    w->cv.notify_all();
+    w->cv.wait(lk, [&] { return w->done; });
+}
+
+static void write_pathlength_groups(PathlengthWriter* w, const mcml_opts* o, const mcml_result* res, int nsets, int S)
+{
+    for (int j = 0; j < nsets; j++)
+        for (int s = 0; s < S; s++) {
+            {
+                std::unique_lock<std::mutex> lk(w->m);
+                const size_t g = (size_t)j * (size_t)S + (size_t)s;
+                w->cv.wait(lk, [&] { return w->finished || (w->have_rows && w->rows >= res->pl_offset[g + 1]); });
+                if (w->finished && w->run_rc != MCML_OK) return;
+            }
+            const int rc = write_pathlength_file(o, res, j, s, nullptr);
+            if (rc != MCML_OK) {
+                w->rc = rc;
+                return;
+            }
+        }
+}
+
+static void write_pathlength_stream(PathlengthWriter* w, const mcml_opts* o, const mcml_result* res, int nsets,
+                                    int S)
+{
+    write_pathlength_groups(w, o, res, nsets, S);
+    {
+        std::lock_guard<std::mutex> lk(w->m);
+        w->done = true;
+    }
+    w->cv.notify_all();
+}
 
 /* Consecutive runs of input lines (first, count) whose absorbance grids fit the budget; a
  * line that alone exceeds it runs by itself. */
@@ -781,6 +865,7 @@ extern "C" int mcml_main(int argc, char** argv)
         std::cout << "-B option only work with -P or -A option!\n";
         return 0;
     }
+    o.skip_records = 1; /* the files need the sums, rows and grids, not the per-photon records */
     mcml_sim* sims = nullptr;
     int ns = 0;
     char msg[1024];
@@ -821,9 +906,38 @@ extern "C" int mcml_main(int argc, char** argv)
     for (size_t bi = 0; bi < batches.size() && rc == MCML_OK; bi++) {
         const int b0 = batches[bi].first, nb = batches[bi].second;
         mcml_result res;
-        rc = mcml_run_batch(ctx, sims + b0, nb, &o, seed + 0x9E3779B97F4A7C15ull * (uint64_t)b0, &res);
+        /* -P: the pathlength files are written while the rest is read back */
+        const bool stream_pl = o.replay && o.pathlength;
+        PathlengthWriter pw;
+        mcml_readback_sink sink = {pathlength_rows_landed, pathlength_release, &pw};
+        std::thread writer;
+        bool pl_written = false; /* the writer thread has written (or will have written) the -P files */
+        if (stream_pl) {
+            try {
+                writer = std::thread(write_pathlength_stream, &pw, &o, &res, nb, sims[b0].num_sds);
+                pl_written = true;
+            } catch (...) { /* no thread to spare: the files are written after the batch */
+            }
+        }
+        rc = mcml_run_batch_sink(ctx, sims + b0, nb, &o, seed + 0x9E3779B97F4A7C15ull * (uint64_t)b0, &res,
+                                 pl_written ? &sink : nullptr);
+        if (pl_written) {
+            {
+                std::lock_guard<std::mutex> lk(pw.m);
+                pw.finished = true;
+                pw.run_rc = rc;
+            }
+            pw.cv.notify_all();
+            writer.join();
+        }
         if (rc != MCML_OK) {
             printf("Error code=%i, %s.\n", rc, mcml_last_error(ctx));
+            break;
+        }
+        if (pw.rc != MCML_OK) {
+            rc = pw.rc;
+            printf("Error code=%i writing outputs\n", rc);
+            mcml_result_free(&res);
             break;
         }
         uint64_t total = 0;
@@ -841,7 +955,7 @@ extern "C" int mcml_main(int argc, char** argv)
             info.replay_time_s = res.ms_replay_wall * 1e-3 * share;
             info.gpu_name = mcml_device_name(ctx);
             printf("finish first run, cost %.2f secs\n", info.sim_time_s);
-            rc = mcml_write_outputs(&sims[i], &o, &res, j, argv[3], nullptr, &info);
+            rc = write_outputs(&sims[i], &o, &res, j, argv[3], nullptr, &info, !pl_written);
             if (rc != MCML_OK) {
                 printf("Error code=%i writing outputs\n", rc);
                 break;

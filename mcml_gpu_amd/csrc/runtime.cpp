@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <thread>
 #include <vector>
 
@@ -28,6 +29,7 @@
 #include "../../include/mcml.h"
 #include "mcml_device.h"
 #include "mcml_launch.h"
+#include "mcml_stream.h"
 #include "mwc_multipliers.h"
 #include "white.h"
 
@@ -55,6 +57,7 @@ constexpr uint64_t kFinalDrain = 16384; /* a drain launch of at most this many p
 constexpr size_t kStageBytes = 128u << 20; /* pinned staging buffer for large read-backs */
 constexpr size_t kSmallStageBytes = 16u << 20; /* ... and the one for mid-sized read-backs */
 constexpr size_t kDirectCopyBytes = 64u << 10;  /* read-backs this small go straight to pageable memory */
+constexpr size_t kStreamChunkBytes = 16u << 20; /* read-back chunk when the caller is told of each one */
 /* replay: at most this many staging copies of the grids' corner, in at most this many bytes */
 constexpr size_t kReplayStageCopies = 64;
 constexpr size_t kReplayStageBytes = 64u << 20;
@@ -220,13 +223,17 @@ void parallel_memcpy(char* dst, const char* src, size_t len)
  * ones (an absorbance grid of a few MB) are one DMA into a pinned buffer and a memcpy: a
  * direct copy into pageable memory moved a 3.2 MB grid at about 2.5 GB/s.  Large ones run
  * through two pinned staging buffers, the DMA of one chunk overlapping the host threads' copy
- * of the previous one into the pageable destination. */
-int copy_d2h(mcml_ctx* c, void* dst, const void* src, size_t bytes)
+ * of the previous one into the pageable destination.  With `landed`, the caller is told the
+ * bytes [0, b) that are in `dst` after every chunk, and chunks are 16 MB instead of 128 MB so
+ * that it can start on the first ones early. */
+int copy_d2h(mcml_ctx* c, void* dst, const void* src, size_t bytes,
+             const std::function<void(size_t)>* landed = nullptr)
 {
     if (bytes == 0) return MCML_OK;
     if (bytes <= kDirectCopyBytes) {
         HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (landed) (*landed)(bytes);
         return MCML_OK;
     }
     if (bytes <= kSmallStageBytes) {
@@ -234,15 +241,17 @@ int copy_d2h(mcml_ctx* c, void* dst, const void* src, size_t bytes)
         HIP_TRY(c, hipMemcpyAsync(c->stage_small, src, bytes, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         parallel_memcpy(static_cast<char*>(dst), static_cast<const char*>(c->stage_small), bytes);
+        if (landed) (*landed)(bytes);
         return MCML_OK;
     }
     for (int i = 0; i < 2; i++) {
         if (!c->stage[i]) HIP_TRY(c, hipHostMalloc(&c->stage[i], kStageBytes, hipHostMallocDefault));
         if (!c->stage_ev[i]) HIP_TRY(c, hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming));
     }
-    const size_t n = (bytes + kStageBytes - 1) / kStageBytes;
+    const size_t chunk = landed ? kStreamChunkBytes : kStageBytes;
+    const size_t n = (bytes + chunk - 1) / chunk;
     auto issue = [&](size_t i) -> hipError_t {
-        const size_t off = i * kStageBytes, len = std::min(kStageBytes, bytes - off);
+        const size_t off = i * chunk, len = std::min(chunk, bytes - off);
         hipError_t e = hipMemcpyAsync(c->stage[i & 1], static_cast<const char*>(src) + off, len,
                                       hipMemcpyDeviceToHost, c->stream);
         return e != hipSuccess ? e : hipEventRecord(c->stage_ev[i & 1], c->stream);
@@ -251,9 +260,10 @@ int copy_d2h(mcml_ctx* c, void* dst, const void* src, size_t bytes)
     if (n > 1) HIP_TRY(c, issue(1));
     for (size_t i = 0; i < n; i++) {
         HIP_TRY(c, hipEventSynchronize(c->stage_ev[i & 1]));
-        const size_t off = i * kStageBytes, len = std::min(kStageBytes, bytes - off);
+        const size_t off = i * chunk, len = std::min(chunk, bytes - off);
         parallel_memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(c->stage[i & 1]), len);
         if (i + 2 < n) HIP_TRY(c, issue(i + 2)); /* into the buffer just drained */
+        if (landed) (*landed)(off + len);
     }
     return MCML_OK;
 }
@@ -1166,8 +1176,8 @@ int replay_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_op
 
 } // namespace
 
-extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, const mcml_opts* opts, uint64_t seed,
-                              mcml_result* out)
+int mcml_run_batch_sink(mcml_ctx* c, const mcml_sim* sims, int nsets, const mcml_opts* opts, uint64_t seed,
+                        mcml_result* out, const mcml_readback_sink* sink)
 {
     if (!c || !sims || !out) return MCML_ERR_ARG;
     memset(out, 0, sizeof(*out));
@@ -1270,41 +1280,53 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
     }
     const double t1 = now_ms();
     out->ms_first_pass_wall = t1 - t0;
+    /* the caller hears of the pathlength rows as they land (whole rows only) */
+    std::function<void(size_t)> rows_fn;
+    const std::function<void(size_t)>* rows_cb = nullptr;
+    bool reported = false;
+    if (sink && sink->rows_landed) {
+        const size_t row_bytes = sizeof(float) * (size_t)(L + 2);
+        rows_fn = [=, &reported](size_t b) {
+            reported = true;
+            sink->rows_landed(sink->user, out, (uint64_t)(b / row_bytes));
+        };
+        rows_cb = &rows_fn;
+    }
+    auto fail = [&](int code) { /* the caller stops reading the rows before they are freed */
+        if (reported && sink->release) sink->release(sink->user);
+        mcml_result_free(out);
+        return code;
+    };
     if (o.replay) {
         out->pl_offset = (uint64_t*)malloc(sizeof(uint64_t) * (groups + 1));
         out->replay_reflectance = (double*)calloc(groups, sizeof(double));
         out->pathlength = (float*)host_alloc(sizeof(float) * (size_t)std::max<uint64_t>(n, 1) * (size_t)(L + 2));
         if (!out->pl_offset || !out->replay_reflectance || !out->pathlength) {
-            mcml_result_free(out);
-            return MCML_ERR_NOMEM;
+            return fail(MCML_ERR_NOMEM);
         }
         memcpy(out->pl_offset, off.data(), sizeof(uint64_t) * (groups + 1));
         const bool grids = o.absorbance != 0;
         if (inline_pl) { /* the first pass already produced the replay rows, in file order */
             memcpy(out->replay_reflectance, out->reflectance, sizeof(double) * groups);
             out->ms_dump_kernel = dump_ms;
-            if ((rc = copy_d2h(c, out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2)))) {
-                mcml_result_free(out);
-                return rc;
+            if ((rc = copy_d2h(c, out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2), rows_cb))) {
+                return fail(rc);
             }
         }
         if (grids && inline_pl && !grids_done && n > 0) { /* the replay pass books the grids only */
             if ((rc = ensure(c, c->rrows, sizeof(float) * (size_t)n * (size_t)(L + 2))) ||
                 (rc = ensure(c, c->rgrp, sizeof(uint64_t) * (groups + 1))) ||
                 (rc = ensure(c, c->rgq, sizeof(unsigned long long) * groups))) {
-                mcml_result_free(out);
-                return rc;
+                return fail(rc);
             }
             if ((rc = replay_pass(c, der, o, seed, n, off, gs, out))) {
-                mcml_result_free(out);
-                return rc;
+                return fail(rc);
             }
         } else if (!inline_pl) { /* absorbance maps replayed in a pass of their own (transport.cu:160-222) */
             if ((rc = ensure(c, c->rrows, sizeof(float) * (size_t)n * (size_t)(L + 2))) ||
                 (rc = ensure(c, c->arz, sizeof(unsigned long long) * rz_cells)) ||
                 (rc = ensure(c, c->a0z, sizeof(unsigned long long) * z_cells))) {
-                mcml_result_free(out);
-                return rc;
+                return fail(rc);
             }
             if (grids) {
                 HIP_TRY(c, hipMemsetAsync(c->arz.p, 0, sizeof(unsigned long long) * rz_cells, c->stream));
@@ -1316,38 +1338,39 @@ extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, cons
                 if ((rc = ensure(c, c->rgrp, sizeof(uint64_t) * (groups + 1))) ||
                     (rc = ensure(c, c->rgq, sizeof(unsigned long long) * groups)) ||
                     (rc = replay_pass(c, der, o, seed, n, off, gs, out))) {
-                    mcml_result_free(out);
-                    return rc;
+                    return fail(rc);
                 }
             }
             std::vector<uint64_t> off2;
             if ((rc = gather_and_reduce(c, n, nsets, S, as<float>(c->rrows), L + 2, nullptr, L + 2, false,
                                         out->replay_reflectance, off2, &out->ms_dump_kernel))) {
-                mcml_result_free(out);
-                return rc;
+                return fail(rc);
             }
-            if ((rc = copy_d2h(c, out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2)))) {
-                mcml_result_free(out);
-                return rc;
+            if ((rc = copy_d2h(c, out->pathlength, c->rows.p, sizeof(float) * (size_t)n * (size_t)(L + 2), rows_cb))) {
+                return fail(rc);
             }
         }
         if (grids) {
             out->A_rz = (uint64_t*)host_alloc(sizeof(uint64_t) * std::max<size_t>(rz_cells, 1));
             out->A0_z = (uint64_t*)malloc(sizeof(uint64_t) * std::max<size_t>(z_cells, 1));
             if (!out->A_rz || !out->A0_z) {
-                mcml_result_free(out);
-                return MCML_ERR_NOMEM;
+                return fail(MCML_ERR_NOMEM);
             }
             if ((rc = copy_d2h(c, out->A_rz, c->arz.p, sizeof(uint64_t) * rz_cells)) ||
                 (rc = copy_d2h(c, out->A0_z, c->a0z.p, sizeof(uint64_t) * z_cells))) {
-                mcml_result_free(out);
-                return rc;
+                return fail(rc);
             }
         }
         out->replayed = 1;
         out->ms_replay_wall = now_ms() - t1;
     }
     return MCML_OK;
+}
+
+extern "C" int mcml_run_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, const mcml_opts* opts, uint64_t seed,
+                              mcml_result* out)
+{
+    return mcml_run_batch_sink(c, sims, nsets, opts, seed, out, nullptr);
 }
 
 extern "C" int mcml_run(mcml_ctx* c, const mcml_sim* sim, const mcml_opts* opts, uint64_t seed, mcml_result* out)

@@ -112,3 +112,43 @@ def test_cli_outputs_do_not_depend_on_batching(tmp_path):
     assert names == sorted(f for f in os.listdir(outs[1]) if f not in ("sim_set.json", "input.txt", "summary.json"))
     for f in names:
         assert (outs[0] / f).read_bytes() == (outs[1] / f).read_bytes(), f
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("binary,photons", [(True, 3000000), (False, 400000)])
+def test_cli_pathlength_files_written_while_reading_back(tmp_path, binary, photons):
+    """-P files are written by a thread of their own as each (set, SDS) group's rows land on
+    the host, 16 MB at a time.  Three accept-all detectors on a semi-infinite medium give
+    about 3 records per two photons (binary case: over 80 MB of rows, several read-back
+    chunks): every file must hold its group's rows, whole and in order, as the library
+    returns them for the same seed."""
+    cfg = {"number_simulation": 1, "number_photons": photons, "number_layers": 3, "detector_reflectance": 0.0,
+           "upper_n": 1.457, "lower_n": 1.457, "source_probe_oblique": 1, "detector_probe_oblique": 1,
+           "probes": {"source": {"NA": 0.0, "radius": 0.0, "angle": 0.0}, "num_SDS": 3,
+                      "detectors": [{"pos": 0.0, "NA": 1.457, "radius": 1000.0, "angle": 0.0}] * 3}}
+    (tmp_path / "sim_set.json").write_text(json.dumps(cfg))
+    (tmp_path / "input.txt").write_text("0.1 0.2 60.0 1.4 0.8 0.3 0.05 120.0 1.37 0.9 0.2 80.0 1.45 0.85\n")
+    args = [M.EXE_PATH, "sim_set.json", "input.txt", "output.txt", "-R", "-P", "-S", str(SEED)]
+    if binary:
+        args.append("-B")
+    print("cli: accept-all, %d photons, binary=%s" % (photons, binary), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    print(r.stdout[-1000:], r.stderr[-1000:], flush=True)
+    assert r.returncode == 0
+    sims = M.read_input(str(tmp_path / "sim_set.json"), str(tmp_path / "input.txt"))
+    with M.Context(0) as ctx:
+        res = ctx.run_batch(sims, M.make_opts(replay=True, pathlength=True, binary=binary), seed=SEED)
+    L = 3
+    if binary:
+        assert len(res.pathlength) * (L + 2) * 4 > 3 * (16 << 20)
+    for s in range(1, 4):
+        want = res.pathlength_records(0, s)
+        if binary:
+            got = np.fromfile(tmp_path / ("pathlength_SDS_%d.bin" % s), dtype="<f4").reshape(-1, L + 2)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), s
+        else:
+            text = (tmp_path / ("pathlength_SDS_%d.txt" % s)).read_text()
+            lines = text.split("\n")
+            assert lines[-1] == "" and len(lines) - 1 == len(want)
+            got = np.array(text.split(), dtype=np.float64).reshape(-1, L + 2)
+            assert np.allclose(got, want, rtol=6e-6, atol=0), s
