@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -1503,21 +1504,37 @@ extern "C" int mcml_average_pathlength(const mcml_sim* sim, const mcml_result* r
     if (rc) return rc;
     const int S = res->num_sds, L = res->num_layers;
     const double inverse_start_weight = double(1 / d.start_weight);
-    for (int s = 0; s < S; s++) {
+    /* one (SDS, layer) pair: its two sums run over the records in order, as in the reference */
+    auto pair = [&](int s, int l) {
         const uint64_t b = res->pl_offset[(size_t)set * S + s], e = res->pl_offset[(size_t)set * S + s + 1];
-        for (int l = 1; l <= L; l++) {
-            double sum_r = 0, sum_c = 0;
-            const double changed_mua = MCML_MUA_CHANGE_RATIO * d.layer[l].mua;
-            for (uint64_t i = b; i < e; i++) {
-                const float* rec = res->pathlength + i * (uint64_t)(L + 2);
-                double energy = rec[0] * inverse_start_weight;
-                sum_r += energy;
-                energy *= exp(-1 * changed_mua * rec[l]);
-                sum_c += energy;
-            }
-            outv[s * L + l - 1] = log(sum_r / sum_c) / changed_mua;
+        double sum_r = 0, sum_c = 0;
+        const double changed_mua = MCML_MUA_CHANGE_RATIO * d.layer[l].mua;
+        for (uint64_t i = b; i < e; i++) {
+            const float* rec = res->pathlength + i * (uint64_t)(L + 2);
+            double energy = rec[0] * inverse_start_weight;
+            sum_r += energy;
+            energy *= exp(-1 * changed_mua * rec[l]);
+            sum_c += energy;
         }
+        outv[s * L + l - 1] = log(sum_r / sum_c) / changed_mua;
+    };
+    /* The S x L pairs are independent; with many records they run on up to 16 host threads
+     * (each pair's sums keep their order, so the values do not depend on the threads). */
+    const int npairs = S * L;
+    const uint64_t nrec = res->pl_offset[(size_t)set * S + S] - res->pl_offset[(size_t)set * S];
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = nrec < 100000 ? 1 : (int)std::min<unsigned>({16u, hw, (unsigned)npairs});
+    std::atomic<int> next(0);
+    auto work = [&] {
+        for (int q; (q = next.fetch_add(1)) < npairs;) pair(q / L, q % L + 1);
+    };
+    std::vector<std::thread> th;
+    try {
+        for (int t = 1; t < nt; t++) th.emplace_back(work);
+    } catch (...) { /* no thread to spare: this thread takes the rest */
     }
+    work();
+    for (std::thread& t : th) t.join();
     return MCML_OK;
 }
 
