@@ -231,9 +231,10 @@ const char* mcml_device_name(const mcml_ctx* ctx);
 int mcml_device_count(int* count);
 
 /* Self-test of the device arithmetic the transport relies on (no counterpart in the
- * reference): the hardware reciprocal and square root with their one-step corrections must
- * give exactly IEEE 1.0f / x (every significand at 16 exponents) and sqrtf(x) (every float in
- * [2^-126, 4)), as the CPU restatement computes them.  mismatches[0], [1] receive the counts.
+ * reference): the hardware reciprocal and reciprocal square root with their one-step
+ * corrections must give exactly IEEE 1.0f / x (every significand at 16 exponents) and
+ * sqrtf(x) (every normal float), as the CPU restatement computes them.  mismatches[0], [1]
+ * receive the counts.
  * mcml_create runs it and fails with MCML_ERR_DEVICE on any mismatch. */
 int mcml_selftest_math(mcml_ctx* ctx, uint64_t* mismatches);
 

@@ -152,12 +152,16 @@ __device__ __forceinline__ void launch(Photon& p, const DevSet* S, uint64_t seed
  * (within 1 ulp) refined by one Newton step e = 1 - x y, y + y e (two fmas) is the correctly
  * rounded 1/x: Markstein's theorem leaves only significands of all ones in doubt, and
  * mcml_selftest_math checks every significand at 16 exponents on the device.  The square
- * root unit's result is corrected to the nearest float by testing its two neighbours with
- * exact fma residuals (the Tuckerman test); below 2^-96, where the residuals underflow, the
- * compiler's IEEE sequence takes over (mcml_selftest_math checks [2^-126, 4) exhaustively).
- * So the transport's spin computes exactly 1.0f / x and sqrtf(x), as its CPU restatement
- * does, in 4 and about 7 issue slots instead of 6 and 11 for Newton iterations from a
- * bit-pattern seed. */
+ * root comes from the reciprocal square root unit, y ~ 1/sqrt(x), with one Markstein
+ * correction: s = x y, h = y / 2, r = x - s^2 (exact by fma), s + r h.  On gfx950 this is
+ * the correctly rounded sqrt for every float from 2^-96 up (checked exhaustively; none of
+ * the bare v_sqrt_f32, v_rcp_f32 results is correctly rounded everywhere: 15 % and 11 % of
+ * the significands differ), and mcml_selftest_math checks every normal float at context
+ * creation; below 2^-96, where r underflows, the compiler's IEEE sequence takes over.  So
+ * the transport computes exactly 1.0f / x and sqrtf(x), as its CPU restatement does, in 3
+ * and 5 VALU instructions (one of them a transcendental each) instead of 6 and 11 for
+ * Newton iterations from a bit-pattern seed, or 9 for the square root unit's result and the
+ * test of its two neighbours. */
 __device__ __forceinline__ float rcp_rn(float x)
 {
     const float y = __builtin_amdgcn_rcpf(x);
@@ -168,12 +172,10 @@ __device__ __forceinline__ float rcp_rn(float x)
 __device__ __forceinline__ float sqrt_rn(float x)
 {
     if (__builtin_expect(x < 0x1p-96f, 0)) return sqrtf(x);
-    float s = __builtin_amdgcn_sqrtf(x);
-    const uint32_t b = __float_as_uint(s);
-    const float sd = __uint_as_float(b - 1u), su = __uint_as_float(b + 1u);
-    const float rd = fmaf(-sd, s, x), ru = fmaf(-su, s, x);
-    s = rd <= 0.0f ? sd : s;
-    return ru > 0.0f ? su : s;
+    const float y = __builtin_amdgcn_rsqf(x);
+    const float s = x * y, h = 0.5f * y;
+    const float r = fmaf(-s, s, x);
+    return fmaf(r, h, s);
 }
 
 /* Henyey-Greenstein deflection + azimuth + renormalisation (Spin, transport.cu:742-786),
@@ -196,7 +198,7 @@ __device__ __forceinline__ void spin(Photon& p)
             const float temp = p.hg_a / D;
             cost = fmaf(-temp, temp, p.hg_d) * p.hg_inv2g;
         }
-        const float sint = sqrtf(fmaf(-cost, cost, 1.0f));
+        const float sint = sqrt_rn(fmaf(-cost, cost, 1.0f));
         p.dx = sint * cosp;
         p.dy = sint * sinp;
         p.dz = copysignf(cost, p.dz * cost);
@@ -253,7 +255,7 @@ __device__ __forceinline__ bool reflect(Photon& p, const DevSet* S, int new_laye
         return false;
     }
     float e = B.z * (1.0f - ci * ci);
-    r = 2.0f * sqrtf((1.0f - ci * ci) * (1.0f - e) * e * ci * ci);
+    r = 2.0f * sqrt_rn((1.0f - ci * ci) * (1.0f - e) * e * ci * ci);
     e = e + (ci * ci) * (1.0f - 2.0f * e);
     r = e * ((1.0f - e - r) / ((1.0f - e + r) * (e + r)));
     if (mcml_uniform(&p.rng) <= r) {
@@ -264,7 +266,7 @@ __device__ __forceinline__ bool reflect(Photon& p, const DevSet* S, int new_laye
     e = r * r * (1.0f - ci * ci);
     p.dx = p.dx * r;
     p.dy = p.dy * r;
-    p.dz = copysignf(sqrtf(1.0f - e), p.dz);
+    p.dz = copysignf(sqrt_rn(1.0f - e), p.dz);
     p.layer = new_layer;
     return false;
 }
@@ -1443,8 +1445,11 @@ extern "C" __global__ void __launch_bounds__(256) mcml_clear_words(unsigned long
 }
 
 /* Self-test of rcp_rn and sqrt_rn against IEEE division and square root: kind 0 checks
- * every significand of x at 16 exponents from 2^-126 to 2^125, kind 1 every float in
- * [2^-126, 4).  Mismatches are counted in *bad. */
+ * every significand of x at 16 exponents from 2^-126 to 2^125, kind 1 every normal float
+ * (bit patterns [SELFTEST_SQRT_LO, SELFTEST_SQRT_HI): 2^-126 up to the largest finite
+ * float).  Mismatches are counted in *bad. */
+#define SELFTEST_SQRT_LO 0x00800000ull
+#define SELFTEST_SQRT_HI 0x7F800000ull
 extern "C" __global__ void __launch_bounds__(256) mcml_selftest_kernel(int kind, unsigned long long* bad)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1455,8 +1460,8 @@ extern "C" __global__ void __launch_bounds__(256) mcml_selftest_kernel(int kind,
         const float x = __uint_as_float(((uint32_t)(exps[i >> 23] + 127) << 23) | (uint32_t)(i & 0x7FFFFFu));
         same = __float_as_uint(rcp_rn(x)) == __float_as_uint(1.0f / x);
     } else {
-        if (i >= 0x40800000ull - 0x00800000ull) return;
-        const float x = __uint_as_float((uint32_t)(0x00800000ull + i));
+        if (i >= SELFTEST_SQRT_HI - SELFTEST_SQRT_LO) return;
+        const float x = __uint_as_float((uint32_t)(SELFTEST_SQRT_LO + i));
         same = __float_as_uint(sqrt_rn(x)) == __float_as_uint(sqrtf(x));
     }
     if (!same) atomicAdd(bad, 1ull);
@@ -1564,7 +1569,7 @@ hipError_t mcml_launch_selftest(unsigned long long* bad, hipStream_t st)
     hipLaunchKernelGGL(mcml_selftest_kernel, dim3((unsigned)((16ull << 23) / 256)), dim3(256), 0, st, 0, bad);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(mcml_selftest_kernel, dim3((unsigned)((0x40800000ull - 0x00800000ull + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(mcml_selftest_kernel, dim3((unsigned)((SELFTEST_SQRT_HI - SELFTEST_SQRT_LO + 255) / 256)), dim3(256), 0,
                        st, 1, bad + 1);
     return hipGetLastError();
 }

@@ -320,7 +320,7 @@ def test_lane_statistics_variant_same_results(ctx):
 @pytest.mark.gpu
 def test_device_reciprocal_and_square_root_are_correctly_rounded(ctx):
     """The spin's reciprocal and square root (hardware units plus one correction) equal IEEE
-    1.0f / x and sqrtf(x) on every value the self-test covers (about 1.2e9)."""
+    1.0f / x and sqrtf(x) on every value the self-test covers (about 2.3e9)."""
     bad = (C.c_uint64 * 2)()
     assert ctx.lib.mcml_selftest_math(ctx.h, bad) == 0
     assert list(bad) == [0, 0]
