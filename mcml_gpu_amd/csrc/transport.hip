@@ -294,7 +294,7 @@ __device__ __forceinline__ uint32_t detect_mask(const Photon& p, const DevSet* S
     }
     const float adz = fabsf(p.dz);
     if (adz < S->cosmin) return 0u;
-    const float dist = sqrtf(p.x * p.x + p.y * p.y);
+    const float dist = sqrt_rn(p.x * p.x + p.y * p.y);
     if (dist < S->dlo || dist > S->dhi) return 0u;
     for (int i = i0; i <= i1; i++) {
         const DevDet D = S->det[i];
@@ -308,7 +308,7 @@ __device__ __forceinline__ float credited(const Photon& p, const DevSet* S, int 
 {
     if (S->det_oblique) return p.w;
     const DevDet D = S->det[i];
-    const float dist = sqrtf(p.x * p.x + p.y * p.y);
+    const float dist = sqrt_rn(p.x * p.x + p.y * p.y);
     float t = (dist * dist + D.pos * D.pos - D.radius * D.radius) / (2.0f * dist * D.pos);
     if (t > 1.0f) t = 1.0f;
     return p.w * mcml_acos(t) * RPI_F;
@@ -347,6 +347,11 @@ __device__ __forceinline__ bool hop(Photon& p, float& s, int& nl)
     }
     return nl != p.layer;
 }
+
+/* The layer beyond the boundary a pending hop stopped at.  A hop is cut at z_min only when
+ * it moves up (dz < 0) and at z_max only when it moves down (z stays in [z_min, z_max]
+ * between hops), and a waiting lane does not move, so the sign of dz tells which. */
+__device__ __forceinline__ int beyond_layer(const Photon& p) { return p.layer + (p.dz > 0.0f ? 1 : -1); }
 
 /* PhotonSurvive (transport.cu:855-870).  The reference applies the step cap in the first
  * pass only; a replayed photon was detected within the cap, so applying it in replay too
@@ -437,7 +442,7 @@ __device__ __forceinline__ int interact(Photon& p, float s, GridLane& gl, const 
                 else
                     atomicAdd(&gl.a0_z[zb], (unsigned long long)w_temp);
             } else {
-                float qr = sqrtf(p.x * p.x + p.y * p.y) * ra->rdr;
+                float qr = sqrt_rn(p.x * p.x + p.y * p.y) * ra->rdr;
                 qr = fminf(fmaxf(qr, 0.0f), (float)(ra->nr - 1));
                 const uint64_t key = ((uint64_t)(uint32_t)zb << 32) | (uint32_t)(int)qr;
                 if (key == gl.rl_key) {
@@ -695,7 +700,6 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
     const int L = a.num_layers;
     Photon p;
     bool alive = false, pending = false;
-    int pend_layer = 0;
     uint32_t set = 0;
     uint64_t key = 0; /* set << 45 | photon index */
     WaveQueue q = {0, 0, false, false};
@@ -759,7 +763,6 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                 key = ((uint64_t)__float_as_uint(col[SP_KEY_HI * cs]) << 32) | __float_as_uint(col[SP_KEY_LO * cs]);
                 const uint32_t pb = __float_as_uint(col[SP_PENDING * cs]);
                 pending = (pb & 1u) != 0u;
-                pend_layer = (int)(pb >> 1);
                 if (GRIDS) { /* a replay in progress continues where it was spilled */
                     rp_mask = __float_as_uint(col[SP_RP_MASK * cs]);
                     rl_key = __float_as_uint(col[SP_RL_KEY * cs]);
@@ -809,7 +812,7 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                     col[SP_PL_LAYER * cs] = __int_as_float(p.pl_layer);
                     col[SP_KEY_LO * cs] = __uint_as_float((uint32_t)key);
                     col[SP_KEY_HI * cs] = __uint_as_float((uint32_t)(key >> 32));
-                    col[SP_PENDING * cs] = __uint_as_float((pending ? 1u : 0u) | ((uint32_t)pend_layer << 1));
+                    col[SP_PENDING * cs] = __uint_as_float(pending ? 1u : 0u);
                     /* written by every variant (no replay: mask 0), as a drain launch of
                      * mcml_first_pass_grids may continue what the main launch spilled */
                     col[SP_RP_MASK * cs] = __uint_as_float(rp_mask);
@@ -844,10 +847,7 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
             int nl = 0;
             bool bnd = false;
             if (act) bnd = hop<PL>(p, s, nl);
-            if (act && bnd) {
-                pending = true;
-                pend_layer = nl;
-            }
+            if (act && bnd) pending = true; /* the layer beyond follows from dz (beyond_layer) */
             if (act && !bnd && s > 0.0f) {
                 const float w_temp = rintf(p.albedo * p.w);
                 p.w -= w_temp;
@@ -865,8 +865,8 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
         if (resolve_now(pending, alive, bnd_min) && pending) {
             pending = false;
             if (!GRIDS || rp_mask == 0) c_bnd++;
-            st = cross<PL>(p, pend_layer, sets + set, GRIDS && rp_mask != 0 ? __ffs(rp_mask) - 1 : 0, mask, pl_lane,
-                           pl_stride);
+            st = cross<PL>(p, beyond_layer(p), sets + set, GRIDS && rp_mask != 0 ? __ffs(rp_mask) - 1 : 0, mask,
+                           pl_lane, pl_stride);
         }
         if (GRIDS && rp_mask != 0 && st != ST_ALIVE) { /* a replay ended: flush its run */
             const int sds = __ffs(rp_mask) - 1;
@@ -1187,7 +1187,6 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
     const int bnd_min = a.bnd_min < 1 ? 1 : a.bnd_min;
     Photon p;
     bool alive = false, pending = false;
-    int pend_layer = 0;
     uint64_t job = 0;
     uint32_t jset = 0, jsds = 1;
     const DevSet* S = a.sets;
@@ -1265,15 +1264,12 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
             int nl = 0;
             bool bnd = false;
             if (act) bnd = hop<true>(p, s, nl);
-            if (act && bnd) {
-                pending = true;
-                pend_layer = nl;
-            }
+            if (act && bnd) pending = true;
             if (act && !bnd) st = interact<true>(p, s, gl, &a, n_int);
         }
         if (resolve_now(pending, alive, bnd_min) && pending) {
             pending = false;
-            st = cross<true>(p, pend_layer, S, (int)jsds, mask, pl_lane, REPLAY_BLOCK);
+            st = cross<true>(p, beyond_layer(p), S, (int)jsds, mask, pl_lane, REPLAY_BLOCK);
         }
         if (st != ST_ALIVE) {
             alive = false;
