@@ -681,12 +681,15 @@ __device__ __forceinline__ void book_drop(const FirstPassArgs& a, const Photon& 
     rl_w = wd;
 }
 
-template <bool PL, bool SUMS, bool STATS = false, bool GRIDS = false>
+/* LDS_SETS: the (small) set table is staged in LDS and read there with LDS instructions;
+ * otherwise it is read from global memory (a large batch).  Both are compiled into each
+ * kernel and chosen by a.lds_sets, so that no access goes through flat addressing. */
+template <bool PL, bool SUMS, bool STATS, bool GRIDS, bool LDS_SETS>
 __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned char* lds_raw)
 {
     const DevSet* sets = a.sets;
     size_t lds_off = 0;
-    if (a.lds_sets) { /* stage the (small) set table in LDS; reads go through one pointer */
+    if (LDS_SETS) {
         const uint4* src = reinterpret_cast<const uint4*>(a.sets);
         uint4* dst = reinterpret_cast<uint4*>(lds_raw);
         const int n16 = (int)(sizeof(DevSet) / 16) * a.nsets;
@@ -936,6 +939,15 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
     }
 }
 
+template <bool PL, bool SUMS, bool STATS = false, bool GRIDS = false>
+__device__ __forceinline__ void first_pass_entry(const FirstPassArgs& a, unsigned char* lds_raw)
+{
+    if (a.lds_sets)
+        first_pass_body<PL, SUMS, STATS, GRIDS, true>(a, lds_raw);
+    else
+        first_pass_body<PL, SUMS, STATS, GRIDS, false>(a, lds_raw);
+}
+
 } // namespace
 
 /* ----------------------------------------------------------------- first pass */
@@ -948,7 +960,7 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass(FirstPassArgs a)
 {
     stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    first_pass_body<false, false>(a, lds_raw);
+    first_pass_entry<false, false>(a, lds_raw);
 }
 
 /* Statistics variants: the same passes counting lane-blocks by state and stamping the
@@ -957,14 +969,14 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_stats(FirstPassArgs 
 {
     stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    first_pass_body<false, false, true>(a, lds_raw);
+    first_pass_entry<false, false, true>(a, lds_raw);
 }
 
 extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_pl_stats(FirstPassArgs a)
 {
     stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    first_pass_body<true, false, true>(a, lds_raw);
+    first_pass_entry<true, false, true>(a, lds_raw);
 }
 
 /* The drain launch of a -R -A pass: it continues the main launch's spilled photons, takes the
@@ -975,7 +987,7 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_grids(FirstPassArgs 
 {
     stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    first_pass_body<true, false, false, true>(a, lds_raw);
+    first_pass_entry<true, false, false, true>(a, lds_raw);
 }
 
 /* The same pass adding each detection to its group's accumulators (no records). */
@@ -983,7 +995,7 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_sums(FirstPassArgs a
 {
     stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    first_pass_body<false, true>(a, lds_raw);
+    first_pass_entry<false, true>(a, lds_raw);
 }
 
 /* The same pass recording per-layer pathlength (replay without absorbance maps needs no
@@ -992,7 +1004,7 @@ extern "C" __global__ void FIRST_PASS_ATTRS mcml_first_pass_pl(FirstPassArgs a)
 {
     stage_tables();
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    first_pass_body<true, false>(a, lds_raw);
+    first_pass_entry<true, false>(a, lds_raw);
 }
 
 /* ------------------------------------------------------- reference launch shape */
