@@ -103,8 +103,7 @@ typedef struct __attribute__((aligned(8))) {
     float a, b;
 } mcml_pair;
 
-#define MCML_LN2_HI 0.693359375f       /* 355/512, exact product with small integers */
-#define MCML_LN2_LO -2.12194440e-4f    /* ln 2 - LN2_HI */
+#define MCML_LN2 0.693147182f /* ln 2 rounded to binary32 */
 
 /* -ln(u) for the uniform u = n 2^-24, n = (x >> 8) + 1 in [1, 2^24], from one 32-bit draw x:
  * the exponential step of the photon walk.  Table-driven (Tang, ACM TOMS 16(4), 1990):
@@ -112,9 +111,11 @@ typedef struct __attribute__((aligned(8))) {
  * bits); with the tabulated c_j ~ 1/m and L_j = -ln(c_j) (mcml_tables.h, `tab` =
  * mcml_lg_table), r = m c_j - 1 (one fma, |r| < 1/256) and
  * ln u = (e - 24) ln 2 + L_j + ln(1 + r),  ln(1 + r) = r + r^2 (r/3 - 1/2)  (error < r^4/4).
- * Over all 2^24 inputs the error is below 7.2e-8 absolute where -ln u < 1 and 8.9e-8
- * relative above (tests/test_detmath.py checks every n).  For u within a few ulp of 1 the
- * sum cancels, so the result is clamped at 0: a step is never negative. */
+ * (e - 24) ln 2 is one fma with ln 2 rounded to binary32: |e - 24| <= 24, so the constant's
+ * error adds at most 4.6e-8 where -ln u is near 16.6 and 1.9e-9 below 1.  Over all 2^24
+ * inputs the error is below 7.0e-8 absolute where -ln u < 1 and 8.9e-8 relative above
+ * (tests/test_detmath.py checks every n).  For u within a few ulp of 1 the sum cancels, so
+ * the result is clamped at 0: a step is never negative. */
 MCML_HD float mcml_neglog_u32(uint32_t x, const mcml_pair* tab)
 {
     const uint32_t n = (x >> 8) + 1u;
@@ -127,7 +128,7 @@ MCML_HD float mcml_neglog_u32(uint32_t x, const mcml_pair* tab)
     const float r = fmaf(m, c, -1.0f);
     const float q = fmaf(r, 0.333333343f, -0.5f);
     const float p = fmaf(q * r, r, r);
-    const float lg = fmaf(le, MCML_LN2_HI, fmaf(le, MCML_LN2_LO, L + p));
+    const float lg = fmaf(le, MCML_LN2, L + p);
     return fmaxf(-lg, 0.0f);
 }
 

@@ -160,8 +160,9 @@ void oracle_launch(const mcml_derived* d, uint64_t seed, uint32_t set, uint64_t 
  *   (D formed from v = 2 - u, the float 1.k the uniform is built from):
  *   t = (1 - g^2) / D = (1 - g^2)((1 - dz^2) r) and Q = P (D r); P = sqrtf(x), the
  *   reciprocal and the square root both correctly rounded
- *   renormalisation by 1 - d/2 + 3d^2/8, d = |u|^2 - 1     (second-order rsqrt about 1;
- *                                                           |d| stays at a few ulp)
+ *   renormalisation by 1 - d/2, d = |u|^2 - 1               (first-order rsqrt about 1:
+ *                                                           |d| stays at a few ulp, so
+ *                                                           3d^2/8 is below 1e-13)
  * Returns cos(theta) for the tests. */
 float oracle_spin(oracle_photon* p, float g)
 {
@@ -202,7 +203,7 @@ float oracle_spin(oracle_photon* p, float g)
         p->dy = cyp;
     }
     float d = fmaf(p->dx, p->dx, fmaf(p->dy, p->dy, fmaf(p->dz, p->dz, -1.0f)));
-    float r = fmaf(d, fmaf(d, 0.375f, -0.5f), 1.0f);
+    float r = fmaf(d, -0.5f, 1.0f);
     p->dx = p->dx * r;
     p->dy = p->dy * r;
     p->dz = p->dz * r;

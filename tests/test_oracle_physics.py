@@ -37,6 +37,25 @@ def test_hg_moments(g):
     assert np.abs(dot - c).max() < 2e-5
 
 
+def test_direction_stays_unit_over_the_step_cap():
+    """The renormalisation after every spin is the first-order (1 + d)^(-1/2) ~ 1 - d/2 with
+    d = |u|^2 - 1: over walks of 10 000 spins (the first-pass step cap) the norm stays within
+    a few ulp of 1, so the dropped 3d^2/8 term never matters."""
+    n = 256
+    rng = np.random.default_rng(5)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    cost = np.zeros(n, np.float32)
+    worst = 0.0
+    for it in range(10000):
+        out = np.zeros((n, 3), np.float32)
+        O.lib().oracle_spin_n(0.9, np.ascontiguousarray(d).ravel(), 1000 + it, cost, out.ravel(), n)
+        d = out
+        if it % 100 == 99:
+            worst = max(worst, float(np.abs(np.linalg.norm(d.astype(np.float64), axis=1) - 1).max()))
+    assert worst < 4e-7, worst
+
+
 def test_spin_along_axis_and_azimuth_uniform():
     n = 200000
     for dz in (1.0, -1.0):
