@@ -54,12 +54,15 @@ MCML_HD uint32_t mcml_mwc_next(mcml_mwc* r)
     return (uint32_t)r->s;
 }
 
-/* Uniform in (0, 1]: the top 23 bits k become 1 - k 2^-23, computed exactly as 2 minus the
- * float 1.k in [1, 2) built from the bits (three integer/float operations, no conversion).
- * Never 0, so -log(u) is finite (the same half-open interval as the reference's
- * generator); the smallest value is 2^-23. */
-/* 1 + k 2^-23 in [1, 2) from the top 23 bits k of a draw: the uniform below is 2 minus it. */
-MCML_HD float mcml_one_plus(uint32_t x) { return mcml_u2f(0x3F800000u | (x >> 9)); }
+/* Uniform in (0, 1]: the low 23 bits k of a draw become 1 - k 2^-23, computed exactly as 2
+ * minus the float 1.k in [1, 2) built from the bits (one and-or and one subtraction, no
+ * conversion).  Never 0, so -log(u) is finite (the same half-open interval as the
+ * reference's generator); the smallest value is 2^-23.  Every bit of a multiply-with-carry
+ * output is a digit bit of the expansion of a fraction with a prime denominator, the low
+ * ones as good as the high ones (tests/test_rng.py checks the uniforms' moments, lag
+ * correlation and chi-square). */
+/* 1 + k 2^-23 in [1, 2) from the low 23 bits k of a draw: the uniform below is 2 minus it. */
+MCML_HD float mcml_one_plus(uint32_t x) { return mcml_u2f(0x3F800000u | (x & 0x007FFFFFu)); }
 
 MCML_HD float mcml_uniform(mcml_mwc* r)
 {

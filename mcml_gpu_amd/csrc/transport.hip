@@ -171,7 +171,9 @@ __device__ __forceinline__ float rcp_rn(float x)
 
 __device__ __forceinline__ float sqrt_rn(float x)
 {
-    if (__builtin_expect(x < 0x1p-96f, 0)) return sqrtf(x);
+    /* below 2^-96 the IEEE sequence; a negative x, which only 1 - cos^2 of a cos(theta)
+     * rounded past +-1 can give, is taken as 0 (the spin's sin(theta) = 0) */
+    if (__builtin_expect(x < 0x1p-96f, 0)) return sqrtf(fmaxf(x, 0.0f));
     const float y = __builtin_amdgcn_rsqf(x);
     const float s = x * y, h = 0.5f * y;
     const float r = fmaf(-s, s, x);
@@ -318,23 +320,18 @@ __device__ __forceinline__ float credited(const Photon& p, const DevSet* S, int 
 enum StepOutcome { ST_ALIVE = 0, ST_DEAD = 1, ST_DETECTED = 2 };
 
 /* Hop (transport.cu:372-395): sample the step, stop it at the layer boundary, move, count
- * the step, clamp z.  Returns true when the hop ended on a boundary (*nl = layer beyond). */
+ * the step, clamp z.  Returns true when the hop ended on a boundary: cut at z_min (moving up)
+ * or at z_max (moving down), the layer beyond being beyond_layer(p).  A hop cut at z_min
+ * lands on z_min and cannot pass z_max, so at most one cut happens; the reference's layer
+ * index would move by -1 and +1 if both did, which "exactly one" restates. */
 template <bool REPLAY>
-__device__ __forceinline__ bool hop(Photon& p, float& s, int& nl)
+__device__ __forceinline__ bool hop(Photon& p, float& s)
 {
     s = mcml_neglog_u32(mcml_mwc_next(&p.rng), LG_TAB) * p.mutr;
-    nl = p.layer;
-    bool cut = false;
-    if (fmaf(s, p.dz, p.z) < p.zmin) {
-        nl--;
-        s = (p.zmin - p.z) / p.dz;
-        cut = true;
-    }
-    if (fmaf(s, p.dz, p.z) > p.zmax) {
-        nl++;
-        s = (p.zmax - p.z) / p.dz;
-        cut = true;
-    }
+    const bool lo = fmaf(s, p.dz, p.z) < p.zmin;
+    if (lo) s = (p.zmin - p.z) / p.dz;
+    const bool hi = fmaf(s, p.dz, p.z) > p.zmax;
+    if (hi) s = (p.zmax - p.z) / p.dz;
     p.x = fmaf(p.dx, s, p.x);
     p.y = fmaf(p.dy, s, p.y);
     p.z = fmaf(p.dz, s, p.z);
@@ -342,11 +339,11 @@ __device__ __forceinline__ bool hop(Photon& p, float& s, int& nl)
     if (REPLAY) p.pl_cur += s; /* transport.cu:529: pathlength of the layer the hop started in */
     /* transport.cu:394-395.  An uncut step lands on the value both tests saw, which lies in
      * [zmin, zmax], so only a step cut at a boundary can need the clamp. */
-    if (cut) {
+    if (lo || hi) {
         if (p.z > p.zmax) p.z = p.zmax;
         if (p.z < p.zmin) p.z = p.zmin;
     }
-    return nl != p.layer;
+    return lo != hi;
 }
 
 /* The layer beyond the boundary a pending hop stopped at.  A hop is cut at z_min only when
@@ -848,9 +845,8 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
              * rather than copying it in and out of each level */
             const bool act = alive && !pending && st == ST_ALIVE;
             float s = 0.0f;
-            int nl = 0;
             bool bnd = false;
-            if (act) bnd = hop<PL>(p, s, nl);
+            if (act) bnd = hop<PL>(p, s);
             if (act && bnd) pending = true; /* the layer beyond follows from dz (beyond_layer) */
             if (act && !bnd && s > 0.0f) {
                 const float w_temp = rintf(p.albedo * p.w);
@@ -1057,14 +1053,13 @@ extern "C" __global__ void __launch_bounds__(256) mcml_refshape_pass(RefShapeArg
         bool live = true;
         for (int ii = 0; ii < a.steps; ii++) {
             float s;
-            int nl;
             uint32_t mask = 0;
             int st = ST_ALIVE;
             /* the persistent kernel's step, flat conditions included */
-            const bool bnd = hop<false>(p, s, nl);
+            const bool bnd = hop<false>(p, s);
             if (bnd) {
                 c_bnd++;
-                st = cross<false>(p, nl, S, 0, mask, nullptr, 0);
+                st = cross<false>(p, beyond_layer(p), S, 0, mask, nullptr, 0);
             }
             if (!bnd && s > 0.0f) {
                 n_int++;
@@ -1274,9 +1269,8 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
         for (int kk = 0; kk < STEPS_PER_CHECK; kk++) { /* as in the first pass */
             const bool act = alive && !pending && st == ST_ALIVE;
             float s = 0.0f;
-            int nl = 0;
             bool bnd = false;
-            if (act) bnd = hop<true>(p, s, nl);
+            if (act) bnd = hop<true>(p, s);
             if (act && bnd) pending = true;
             if (act && !bnd) st = interact<true>(p, s, gl, &a, n_int);
         }

@@ -159,7 +159,9 @@ void oracle_launch(const mcml_derived* d, uint64_t seed, uint32_t set, uint64_t 
  *   with one reciprocal r = 1 / (D (1 - dz^2)), D = 1 - g + 2gu, for both divisions
  *   (D formed from v = 2 - u, the float 1.k the uniform is built from):
  *   t = (1 - g^2) / D = (1 - g^2)((1 - dz^2) r) and Q = P (D r); P = sqrtf(x), the
- *   reciprocal and the square root both correctly rounded
+ *   reciprocal and the square root both correctly rounded; at u = 0 or 1 the inversion can
+ *   round cos(theta) past +-1, where the reference's sqrtf(1 - cos^2) is NaN: here
+ *   sin(theta) is 0 (transport.cu:755-759)
  *   renormalisation by 1 - d/2, d = |u|^2 - 1               (first-order rsqrt about 1:
  *                                                           |d| stays at a few ulp, so
  *                                                           3d^2/8 is below 1e-13)
@@ -181,7 +183,7 @@ float oracle_spin(oracle_photon* p, float g)
             float temp = (1.0f - g * g) / D;
             cost = fmaf(-temp, temp, 1.0f + g * g) * (1.0f / (2.0f * g));
         }
-        float sint = sqrtf(fmaf(-cost, cost, 1.0f));
+        float sint = sqrtf(fmaxf(fmaf(-cost, cost, 1.0f), 0.0f)); /* cos(theta) may round past +-1 */
         p->dx = sint * cosp;
         p->dy = sint * sinp;
         p->dz = copysignf(cost, p->dz * cost);
@@ -194,7 +196,7 @@ float oracle_spin(oracle_photon* p, float g)
             cost = fmaf(-temp, temp, 1.0f + g * g) * (1.0f / (2.0f * g));
         }
         float ab = fmaf(-cost, cost, 1.0f) * b;
-        float P = sqrtf(ab);
+        float P = sqrtf(fmaxf(ab, 0.0f)); /* cos(theta) may round past +-1: sin(theta) = 0 */
         float Q = P * (D * r);
         float cxp = fmaf(Q, fmaf(p->dx * p->dz, cosp, -(p->dy * sinp)), p->dx * cost);
         float cyp = fmaf(Q, fmaf(p->dy * p->dz, cosp, p->dx * sinp), p->dy * cost);

@@ -22,7 +22,7 @@ def test_recurrence_matches_bigint_restatement():
             t = mult * (state & 0xFFFFFFFF) + (state >> 32)  # x' = a x + c, carry = high word
             state = t
             x = t & 0xFFFFFFFF
-            exp.append(1.0 - (x >> 9) / 2.0 ** 23)  # 2 - 1.k, exact
+            exp.append(1.0 - (x & 0x7FFFFF) / 2.0 ** 23)  # 2 - 1.k of the low 23 bits, exact
         assert np.array_equal(got, np.array(exp, np.float32))
 
 
