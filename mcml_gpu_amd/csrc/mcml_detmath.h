@@ -168,21 +168,27 @@ MCML_HD void mcml_sincos_turns(float t, float* sn, float* cs)
 }
 
 /* sin and cos of the azimuth 2 pi x / 2^32 for one 32-bit draw x.  Table-driven: the top
- * eight bits j pick (cos, sin) of 2 pi j / 256 (mcml_tables.h, `tab` = mcml_sc_table), the
- * low 24 bits give the remaining angle d < 2 pi / 256 = 0.0246, and angle addition with
- * sin d = d - d^3/6 (error < d^5/120 = 7.4e-11) and cos d = 1 - d^2/2 (error < d^4/24 =
- * 1.5e-8) gives both results within 1.2e-7 (tests/test_detmath.py). */
+ * eight bits j pick (c, s) = (cos, sin) of 2 pi j / 256 (mcml_tables.h, `tab` =
+ * mcml_sc_table), the low 24 bits rho give the remaining angle d = K rho < 2 pi / 256 =
+ * 0.0246 (K = 2 pi / 2^32), and the angle addition reads
+ *   cos(a + d) = c - s sin d - c (1 - cos d),  sin(a + d) = s + c sin d - s (1 - cos d)
+ * with sin d = rho (K - (K^3/6) rho^2) (error < d^5/120 = 7.4e-11) and 1 - cos d =
+ * (K^2/2) rho^2 (error < d^4/24 = 1.5e-8): four operations for the two polynomials and two
+ * fmas for each result.  Both results are within 1.1e-7 of float64 (tests/test_detmath.py). */
+#define MCML_SC_K1 0x1.921fb6p-30f /* 2 pi / 2^32 */
+#define MCML_SC_K2 0x1.3bd3ccp-60f /* (2 pi / 2^32)^2 / 2 */
+#define MCML_SC_K3 0x1.4abbcep-91f /* (2 pi / 2^32)^3 / 6 */
 MCML_HD void mcml_sincos_u32(uint32_t x, const mcml_pair* tab, float* sn, float* cs)
 {
     const uint32_t j = x >> 24;
-    const float d = (float)(x & 0x00FFFFFFu) * 1.46291807926715968e-9f; /* 2 pi / 2^32 */
-    const float d2 = d * d;
-    const float sd = fmaf(d * -0.166666672f, d2, d);
-    const float cd = fmaf(d2, -0.5f, 1.0f);
+    const float rho = (float)(x & 0x00FFFFFFu);
+    const float r2 = rho * rho;
+    const float sd = rho * fmaf(-MCML_SC_K3, r2, MCML_SC_K1); /* sin d */
+    const float vd = MCML_SC_K2 * r2;                         /* 1 - cos d */
     const mcml_pair t = tab[j];
     const float ca = t.a, sa = t.b;
-    *sn = fmaf(sa, cd, ca * sd);
-    *cs = fmaf(ca, cd, -(sa * sd));
+    *sn = fmaf(ca, sd, fmaf(-sa, vd, sa));
+    *cs = fmaf(-sa, sd, fmaf(-ca, vd, ca));
 }
 
 /* sin and cos of an angle in radians, |a| <= 4 (used for launch angles only). */
