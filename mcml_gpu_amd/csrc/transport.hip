@@ -216,8 +216,12 @@ __device__ __forceinline__ void spin(Photon& p)
         const float ab = fmaf(-cost, cost, 1.0f) * b;
         const float P = sqrt_rn(ab);
         const float Q = P * (D * r);
-        const float cxp = fmaf(Q, fmaf(p.dx * p.dz, cosp, -(p.dy * sinp)), p.dx * cost);
-        const float cyp = fmaf(Q, fmaf(p.dy * p.dz, cosp, p.dx * sinp), p.dy * cost);
+        /* ux' = ux (cos + uz Q cos psi) - uy Q sin psi, uy' = uy (cos + uz Q cos psi) +
+         * ux Q sin psi: the bracket is shared */
+        const float Qc = Q * cosp, Qs = Q * sinp;
+        const float k = fmaf(p.dz, Qc, cost);
+        const float cxp = fmaf(p.dx, k, -(Qs * p.dy));
+        const float cyp = fmaf(p.dy, k, Qs * p.dx);
         p.dz = fmaf(-P, cosp, p.dz * cost);
         p.dx = cxp;
         p.dy = cyp;

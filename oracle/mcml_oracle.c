@@ -198,8 +198,12 @@ float oracle_spin(oracle_photon* p, float g)
         float ab = fmaf(-cost, cost, 1.0f) * b;
         float P = sqrtf(fmaxf(ab, 0.0f)); /* cos(theta) may round past +-1: sin(theta) = 0 */
         float Q = P * (D * r);
-        float cxp = fmaf(Q, fmaf(p->dx * p->dz, cosp, -(p->dy * sinp)), p->dx * cost);
-        float cyp = fmaf(Q, fmaf(p->dy * p->dz, cosp, p->dx * sinp), p->dy * cost);
+        /* MCML's update (Wang et al. 1995) with the shared bracket cos + uz Q cos(psi):
+         * ux' = ux (cos + uz Q cos psi) - uy Q sin psi, uy' = uy (...) + ux Q sin psi */
+        float Qc = Q * cosp, Qs = Q * sinp;
+        float k = fmaf(p->dz, Qc, cost);
+        float cxp = fmaf(p->dx, k, -(Qs * p->dy));
+        float cyp = fmaf(p->dy, k, Qs * p->dx);
         p->dz = fmaf(-P, cosp, p->dz * cost);
         p->dx = cxp;
         p->dy = cyp;
