@@ -294,11 +294,12 @@ void fill_devset(const mcml_derived& d, DevSet& S)
         o.mua = in.mua;
         o.albedo = in.mua * in.mutr;
         o.g = in.g;
-        o.hg_a = 1.0f - in.g * in.g;
+        o.hg_inv2g = 1.0f / (2.0f * in.g);
+        const float hg_a = 1.0f - in.g * in.g;
+        o.hg_a2 = (hg_a * hg_a) * o.hg_inv2g;
         o.hg_e = 2.0f * (2.0f * in.g) + (1.0f - in.g);
         o.hg_cn = -(2.0f * in.g);
-        o.hg_d = 1.0f + in.g * in.g;
-        o.hg_inv2g = 1.0f / (2.0f * in.g);
+        o.hg_c0 = (1.0f + in.g * in.g) * o.hg_inv2g;
         o.n = in.n;
         if (l >= 1 && l <= d.num_layers) {
             const float n1 = in.n;

@@ -82,7 +82,7 @@ struct Photon {
     int32_t first;
     mcml_mwc rng;
     /* the current layer's parameters, reloaded only when the layer changes */
-    float zmin, zmax, mutr, albedo, hg_a, hg_e, hg_cn, hg_d, hg_inv2g; /* hg_cn = -2g: 0 = isotropic */
+    float zmin, zmax, mutr, albedo, hg_a2, hg_e, hg_cn, hg_c0; /* hg_cn = -2g: 0 = isotropic */
     /* running pathlength in the current layer; the per-layer totals live in LDS and are
      * exchanged with this register only when the photon changes layer */
     float pl_cur;
@@ -97,11 +97,10 @@ __device__ __forceinline__ void load_layer(Photon& p, const DevSet* S)
     p.zmax = a.y;
     p.mutr = a.z;
     p.albedo = a.w;
-    p.hg_a = b.y;
+    p.hg_a2 = b.y;
     p.hg_e = b.z;
     p.hg_cn = b.w;
-    p.hg_d = c.x;
-    p.hg_inv2g = c.y;
+    p.hg_c0 = c.x;
 }
 
 /* Launch photon k of set `set` (LaunchPhoton, transport.cu:624-719). */
@@ -198,8 +197,8 @@ __device__ __forceinline__ void spin(Photon& p)
     if (b == 0.0f) {
         float cost = fmaf(-2.0f, v, 3.0f);
         if (!iso) {
-            const float temp = p.hg_a / D;
-            cost = fmaf(-temp, temp, p.hg_d) * p.hg_inv2g;
+            const float t = 1.0f / D;
+            cost = fmaf(-(t * t), p.hg_a2, p.hg_c0);
         }
         const float sint = sqrt_rn(fmaf(-cost, cost, 1.0f));
         p.dx = sint * cosp;
@@ -210,8 +209,8 @@ __device__ __forceinline__ void spin(Photon& p)
         const float r = rcp_rn(D * b);
         float cost = fmaf(-2.0f, v, 3.0f);
         if (!iso) {
-            const float temp = p.hg_a * (b * r);
-            cost = fmaf(-temp, temp, p.hg_d) * p.hg_inv2g;
+            const float br = b * r; /* 1 / D */
+            cost = fmaf(-(br * br), p.hg_a2, p.hg_c0);
         }
         const float ab = fmaf(-cost, cost, 1.0f) * b;
         const float P = sqrt_rn(ab);

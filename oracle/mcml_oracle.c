@@ -153,12 +153,15 @@ void oracle_launch(const mcml_derived* d, uint64_t seed, uint32_t set, uint64_t 
 /* Spin, transport.cu:742-786: Henyey-Greenstein (or isotropic for g == 0) deflection,
  * uniform azimuth, MCML direction update with the |dz| == 1 special case, renormalise.
  * Same mathematics, fewer divisions and square roots than the reference's expression:
- *   cos(theta) = (1 + g^2 - t^2) * (1 / 2g)              (reference divides by 2g)
+ *   cos(theta) = c0 - a2 (1/D)^2, c0 = (1 + g^2)(1/2g), a2 = (1 - g^2)^2 (1/2g)
+ *                                                         (= (1 + g^2 - t^2) / 2g,
+ *                                                          t = (1 - g^2) / D, the
+ *                                                          reference's expression)
  *   P = sqrt((1 - cos^2)(1 - dz^2)), Q = P / (1 - dz^2)    (= sin(theta) * sqrt(1-dz^2) and
  *                                                           sin(theta) / sqrt(1-dz^2))
  *   with one reciprocal r = 1 / (D (1 - dz^2)), D = 1 - g + 2gu, for both divisions
  *   (D formed from v = 2 - u, the float 1.k the uniform is built from):
- *   t = (1 - g^2) / D = (1 - g^2)((1 - dz^2) r) and Q = P (D r); P = sqrtf(x), the
+ *   1/D = (1 - dz^2) r and Q = P (D r); P = sqrtf(x), the
  *   reciprocal and the square root both correctly rounded; at u = 0 or 1 the inversion can
  *   round cos(theta) past +-1, where the reference's sqrtf(1 - cos^2) is NaN: here
  *   sin(theta) is 0 (transport.cu:755-759)
@@ -173,6 +176,11 @@ float oracle_spin(oracle_photon* p, float g)
     const float v = mcml_one_plus(mcml_mwc_next(&p->rng));
     const int iso = g == 0.0f;
     const float D = fmaf(-(2.0f * g), v, 2.0f * (2.0f * g) + (1.0f - g));
+    /* cos(theta) = (1 + g^2 - ((1 - g^2) / D)^2) / (2g) = c0 - a2 / D^2, with the per-layer
+     * constants formed as the library forms them (runtime.cpp fill_devset) */
+    const float inv2g = 1.0f / (2.0f * g);
+    const float hg_a = 1.0f - g * g;
+    const float a2 = (hg_a * hg_a) * inv2g, c0 = (1.0f + g * g) * inv2g;
     float sinp, cosp;
     mcml_sincos_u32(mcml_mwc_next(&p->rng), mcml_sc_table, &sinp, &cosp);
     float b = fmaf(-p->dz, p->dz, 1.0f);
@@ -180,8 +188,8 @@ float oracle_spin(oracle_photon* p, float g)
     if (b == 0.0f) { /* travelling along z */
         cost = fmaf(-2.0f, v, 3.0f);
         if (!iso) {
-            float temp = (1.0f - g * g) / D;
-            cost = fmaf(-temp, temp, 1.0f + g * g) * (1.0f / (2.0f * g));
+            float t = 1.0f / D;
+            cost = fmaf(-(t * t), a2, c0);
         }
         float sint = sqrtf(fmaxf(fmaf(-cost, cost, 1.0f), 0.0f)); /* cos(theta) may round past +-1 */
         p->dx = sint * cosp;
@@ -192,8 +200,8 @@ float oracle_spin(oracle_photon* p, float g)
         float r = 1.0f / (D * b);
         cost = fmaf(-2.0f, v, 3.0f);
         if (!iso) {
-            float temp = (1.0f - g * g) * (b * r);
-            cost = fmaf(-temp, temp, 1.0f + g * g) * (1.0f / (2.0f * g));
+            float br = b * r; /* 1 / D */
+            cost = fmaf(-(br * br), a2, c0);
         }
         float ab = fmaf(-cost, cost, 1.0f) * b;
         float P = sqrtf(fmaxf(ab, 0.0f)); /* cos(theta) may round past +-1: sin(theta) = 0 */
