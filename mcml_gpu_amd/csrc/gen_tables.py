@@ -6,12 +6,13 @@ mcml_sc_table[j] = {cos(2 pi j / 256), sin(2 pi j / 256)}, j = 0..255:
     remaining angle below 2 pi / 256).
 
 mcml_lg_table[j] = {c_j, L_j}, j = 0..127, with
-    c_j = binary32(1 / (1 + (j + 1/2) / 128))   (reciprocal of the centre of mantissa cell j)
-    L_j = binary32(-ln(c_j))                    (logarithm of the rounded reciprocal)
+    c_j = binary32(1 / (1 + j / 128))   (reciprocal of the start of mantissa cell j)
+    L_j = binary32(-ln(c_j))            (logarithm of the rounded reciprocal)
     the table of mcml_neglog_u32, after Tang, "Table-driven implementation of the
     logarithm function in IEEE floating-point arithmetic", ACM TOMS 16(4), 1990: for a
-    mantissa m in cell j, r = m c_j - 1 is small (|r| < 1/256) and
-    ln m = L_j + ln(1 + r).
+    mantissa m in cell j, r = m c_j - 1 is small (0 <= r < 1/128 up to rounding) and
+    ln m = L_j + ln(1 + r).  Cell 0 has c_0 = 1 and L_0 = 0, so u = 1 gives -ln u = 0
+    exactly.
 
 Every entry is the double-precision value rounded to nearest binary32, so the table is a
 function of the formulas alone; tests/test_detmath.py recomputes it and compares bits.
@@ -42,7 +43,7 @@ def sc_entries():
 def lg_entries():
     out = []
     for j in range(LG_N):
-        c = f32(1.0 / (1.0 + (j + 0.5) / LG_N))
+        c = f32(1.0 / (1.0 + j / LG_N))
         out.append((c, f32(-math.log(c))))
     return out
 

@@ -111,14 +111,15 @@ typedef struct __attribute__((aligned(8))) {
 /* -ln(u) for the uniform u = n 2^-24, n = (x >> 8) + 1 in [1, 2^24], from one 32-bit draw x:
  * the exponential step of the photon walk.  Table-driven (Tang, ACM TOMS 16(4), 1990):
  * float(n) = 2^e m is exact, m in [1, 2) lies in cell j of 128 (its top seven fraction
- * bits); with the tabulated c_j ~ 1/m and L_j = -ln(c_j) (mcml_tables.h, `tab` =
- * mcml_lg_table), r = m c_j - 1 (one fma, |r| < 1/256) and
- * ln u = (e - 24) ln 2 + L_j + ln(1 + r),  ln(1 + r) = r + r^2 (r/3 - 1/2)  (error < r^4/4).
- * (e - 24) ln 2 is one fma with ln 2 rounded to binary32: |e - 24| <= 24, so the constant's
- * error adds at most 4.6e-8 where -ln u is near 16.6 and 1.9e-9 below 1.  Over all 2^24
- * inputs the error is below 7.0e-8 absolute where -ln u < 1 and 8.9e-8 relative above
- * (tests/test_detmath.py checks every n).  For u within a few ulp of 1 the sum cancels, so
- * the result is clamped at 0: a step is never negative. */
+ * bits); with the tabulated c_j ~ 1 / (1 + j/128) and L_j = -ln(c_j) (mcml_tables.h, `tab` =
+ * mcml_lg_table), r = m c_j - 1 (one fma, 0 <= r < 1/128 up to rounding) and
+ * ln u = (e - 24) ln 2 + L_j + ln(1 + r),  ln(1 + r) = r (1 + r (r/3 - 1/2))  (error < r^4/4
+ * < 1e-9), evaluated as three fmas ending on L_j.  (e - 24) ln 2 is one fma with ln 2
+ * rounded to binary32: |e - 24| <= 24, so the constant adds at most 4.6e-8 where -ln u is
+ * near 16.6 and 1.9e-9 below 1.  Over all 2^24 inputs the error is below 6.3e-8 absolute
+ * where -ln u < 1 and 8.7e-8 relative above (tests/test_detmath.py checks every n), and u = 1
+ * (cell 0: c_0 = 1, L_0 = 0, r = 0) gives exactly 0.  The result is clamped at 0: a step is
+ * never negative. */
 MCML_HD float mcml_neglog_u32(uint32_t x, const mcml_pair* tab)
 {
     const uint32_t n = (x >> 8) + 1u;
@@ -130,8 +131,7 @@ MCML_HD float mcml_neglog_u32(uint32_t x, const mcml_pair* tab)
     const float c = t.a, L = t.b;
     const float r = fmaf(m, c, -1.0f);
     const float q = fmaf(r, 0.333333343f, -0.5f);
-    const float p = fmaf(q * r, r, r);
-    const float lg = fmaf(le, MCML_LN2, L + p);
+    const float lg = fmaf(le, MCML_LN2, fmaf(r, fmaf(q, r, 1.0f), L));
     return fmaxf(-lg, 0.0f);
 }
 
