@@ -118,8 +118,8 @@ typedef struct __attribute__((aligned(8))) {
  * rounded to binary32: |e - 24| <= 24, so the constant adds at most 4.6e-8 where -ln u is
  * near 16.6 and 1.9e-9 below 1.  Over all 2^24 inputs the error is below 6.3e-8 absolute
  * where -ln u < 1 and 8.7e-8 relative above (tests/test_detmath.py checks every n), and u = 1
- * (cell 0: c_0 = 1, L_0 = 0, r = 0) gives exactly 0.  The result is clamped at 0: a step is
- * never negative. */
+ * (cell 0: c_0 = 1, L_0 = 0, r = 0) gives exactly 0.  No input gives a positive logarithm
+ * (every one of the 2^24 is checked), so a step is never negative without a clamp. */
 MCML_HD float mcml_neglog_u32(uint32_t x, const mcml_pair* tab)
 {
     const uint32_t n = (x >> 8) + 1u;
@@ -132,7 +132,7 @@ MCML_HD float mcml_neglog_u32(uint32_t x, const mcml_pair* tab)
     const float r = fmaf(m, c, -1.0f);
     const float q = fmaf(r, 0.333333343f, -0.5f);
     const float lg = fmaf(le, MCML_LN2, fmaf(r, fmaf(q, r, 1.0f), L));
-    return fmaxf(-lg, 0.0f);
+    return -lg;
 }
 
 /* ---------------------------------------------------------------- sin / cos */
