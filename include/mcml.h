@@ -238,7 +238,8 @@ int mcml_device_count(int* count);
  * mcml_create runs it and fails with MCML_ERR_DEVICE on any mismatch. */
 int mcml_selftest_math(mcml_ctx* ctx, uint64_t* mismatches);
 
-/* Derive the per-simulation constants (io.cpp:179-237). */
+/* Derive the per-simulation constants (io.cpp:179-237).  A negative or NaN layer thickness
+ * is rejected with MCML_ERR_INPUT (the reference reads it unchecked). */
 int mcml_derive(const mcml_sim* sim, mcml_derived* out);
 
 /* One simulation: replaces DoOneSimulation (transport.cu:57-272) minus the file output.

@@ -108,6 +108,17 @@ typedef struct __attribute__((aligned(8))) {
 
 #define MCML_LN2 0.693147182f /* ln 2 rounded to binary32 */
 
+/* 0x4B000000 | (b >> 23): the funnel shift (0x00258000:b) >> 23 on the GPU (one v_alignbit),
+ * the same bits on the host */
+MCML_HD uint32_t mcml_exp_magic(uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbit(0x00258000u, b, 23u);
+#else
+    return 0x4B000000u | (b >> 23);
+#endif
+}
+
 /* -ln(u) for the uniform u = n 2^-24, n = (x >> 8) + 1 in [1, 2^24], from one 32-bit draw x:
  * the exponential step of the photon walk.  Table-driven (Tang, ACM TOMS 16(4), 1990):
  * float(n) = 2^e m is exact, m in [1, 2) lies in cell j of 128 (its top seven fraction
@@ -126,7 +137,9 @@ MCML_HD float mcml_neglog_u32(uint32_t x, const mcml_pair* tab)
     const uint32_t b = mcml_f2u((float)n);
     const uint32_t j = (b >> 16) & 127u;
     const float m = mcml_u2f((b & 0x007FFFFFu) | 0x3F800000u);
-    const float le = (float)((int32_t)(b >> 23) - 151);
+    /* e - 24 = (biased exponent) - 151, exactly: 2^23 + biased exponent as a float (its bits
+     * 0x4B000000 | b >> 23, one funnel shift on the GPU) minus 2^23 + 151 */
+    const float le = mcml_u2f(mcml_exp_magic(b)) - 8388759.0f;
     const mcml_pair t = tab[j];
     const float c = t.a, L = t.b;
     const float r = fmaf(m, c, -1.0f);

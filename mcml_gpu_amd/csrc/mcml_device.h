@@ -19,7 +19,8 @@ struct alignas(16) DevLayer {
     float g, hg_a2, hg_e, hg_cn;       /* ((1 - g*g) * (1 - g*g)) * (1 / (2*g)),
                                           2*(2*g) + (1 - g), -(2*g): the HG denominator
                                           1 - g + 2gu is D = hg_e + hg_cn * (2 - u), and
-                                          cos(theta) = hg_c0 - hg_a2 / D^2 */
+                                          cos(theta) = hg_c0 - hg_a2 / D^2; for g = 0
+                                          hg_a2 = -2, hg_c0 = 3 (cos(theta) = 3 - 2v) */
     float hg_c0, hg_inv2g, n, mua;     /* (1 + g*g) * (1 / (2*g)), 1 / (2*g) */
     float up_n, up_ratio, up_ratio2, up_rnorm; /* n2, n1/n2, (n1*n1)/(n2*n2), (n1-n2)/(n1+n2) */
     float dn_n, dn_ratio, dn_ratio2, dn_rnorm;

@@ -300,6 +300,10 @@ void fill_devset(const mcml_derived& d, DevSet& S)
         o.hg_e = 2.0f * (2.0f * in.g) + (1.0f - in.g);
         o.hg_cn = -(2.0f * in.g);
         o.hg_c0 = (1.0f + in.g * in.g) * o.hg_inv2g;
+        if (in.g == 0.0f) { /* isotropic: cos(theta) = hg_c0 + hg_a2 * v = 3 - 2v for the draw's 1.k */
+            o.hg_a2 = -2.0f;
+            o.hg_c0 = 3.0f;
+        }
         o.n = in.n;
         if (l >= 1 && l <= d.num_layers) {
             const float n1 = in.n;
@@ -421,6 +425,7 @@ extern "C" int mcml_derive(const mcml_sim* s, mcml_derived* d)
     for (int l = 1; l <= L; l++) {
         const mcml_layer& in = s->layers[l - 1];
         const float h = (l < L) ? in.thickness : 0.0f;
+        if (!(h >= 0.0f)) return MCML_ERR_INPUT; /* the transport relies on z_min <= z_max */
         z_hi += h;
         mcml_layer_derived& o = d->layer[l];
         o.z_min = z_lo;
@@ -587,6 +592,16 @@ extern "C" void mcml_result_free(mcml_result* r)
 /* ====================================================================== runs */
 namespace {
 
+/* Reports why mcml_derive rejected set j and returns its code. */
+int derive_error(mcml_ctx* c, int j, int rc)
+{
+    if (rc == MCML_ERR_INPUT)
+        set_err(c, "set %d: a layer thickness is negative or NaN", j);
+    else
+        set_err(c, "set %d: layers must be 1..%d and SDS 1..%d", j, MCML_MAX_LAYERS, MCML_MAX_SDS);
+    return rc;
+}
+
 int validate_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, std::vector<mcml_derived>& der)
 {
     if (nsets < 1) {
@@ -600,10 +615,7 @@ int validate_batch(mcml_ctx* c, const mcml_sim* sims, int nsets, std::vector<mcm
     der.resize((size_t)nsets);
     for (int j = 0; j < nsets; j++) {
         int rc = mcml_derive(&sims[j], &der[(size_t)j]);
-        if (rc != MCML_OK) {
-            set_err(c, "set %d: layers must be 1..%d and SDS 1..%d", j, MCML_MAX_LAYERS, MCML_MAX_SDS);
-            return rc;
-        }
+        if (rc != MCML_OK) return derive_error(c, j, rc);
         if (der[(size_t)j].num_layers != der[0].num_layers || der[(size_t)j].num_sds != der[0].num_sds) {
             set_err(c, "set %d: all sets of a batch must share number_layers and num_SDS", j);
             return MCML_ERR_ARG;
@@ -1419,10 +1431,7 @@ extern "C" int mcml_run_batch_sums_noalloc(mcml_ctx* c, const mcml_sim* sims, in
     for (int j = 0; j < nsets; j++) {
         mcml_derived d;
         const int rc = mcml_derive(&sims[j], &d);
-        if (rc != MCML_OK) {
-            set_err(c, "set %d: layers must be 1..%d and SDS 1..%d", j, MCML_MAX_LAYERS, MCML_MAX_SDS);
-            return rc;
-        }
+        if (rc != MCML_OK) return derive_error(c, j, rc);
         if (j == 0) {
             L = d.num_layers;
             S = d.num_sds;

@@ -170,13 +170,25 @@ __device__ __forceinline__ float rcp_rn(float x)
 
 __device__ __forceinline__ float sqrt_rn(float x)
 {
-    /* below 2^-96 the IEEE sequence; a negative x, which only 1 - cos^2 of a cos(theta)
-     * rounded past +-1 can give, is taken as 0 (the spin's sin(theta) = 0) */
+    /* below 2^-96 the IEEE sequence, with a negative x taken as 0 */
     if (__builtin_expect(x < 0x1p-96f, 0)) return sqrtf(fmaxf(x, 0.0f));
     const float y = __builtin_amdgcn_rsqf(x);
     const float s = x * y, h = 0.5f * y;
     const float r = fmaf(-s, s, x);
     return fmaf(r, h, s);
+}
+
+/* sqrt_rn for the spin's arguments, (1 - cos^2) and (1 - cos^2)(1 - dz^2).  Each factor is
+ * 1 - c^2 formed by one fma from a float c: 0 at |c| = 1 and otherwise at least 2^-24 in
+ * magnitude (the floats next to 1 are 1 - 2^-24 and 1 + 2^-23), so the argument is 0 or at
+ * least 2^-48 in magnitude, never in (0, 2^-96) where sqrt_rn takes its IEEE branch.  A
+ * non-positive argument (c rounded to or past +-1) gives sin(theta) = 0 by a select. */
+__device__ __forceinline__ float sqrt_spin(float x)
+{
+    const float y = __builtin_amdgcn_rsqf(x);
+    const float s = x * y, h = 0.5f * y;
+    const float r = fmaf(-s, s, x);
+    return x > 0.0f ? fmaf(r, h, s) : 0.0f;
 }
 
 /* Henyey-Greenstein deflection + azimuth + renormalisation (Spin, transport.cu:742-786),
@@ -194,26 +206,22 @@ __device__ __forceinline__ void spin(Photon& p)
     float sinp, cosp;
     mcml_sincos_u32(mcml_mwc_next(&p.rng), SC_TAB, &sinp, &cosp);
     const float b = fmaf(-p.dz, p.dz, 1.0f);
+    /* cos(theta) = fma(X, hg_a2, hg_c0): X = -1/D^2, or for an isotropic layer X = v with
+     * hg_a2 = -2, hg_c0 = 3 (3 - 2v, one rounding) */
     if (b == 0.0f) {
-        float cost = fmaf(-2.0f, v, 3.0f);
-        if (!iso) {
-            const float t = 1.0f / D;
-            cost = fmaf(-(t * t), p.hg_a2, p.hg_c0);
-        }
-        const float sint = sqrt_rn(fmaf(-cost, cost, 1.0f));
+        const float t = 1.0f / D;
+        const float cost = fmaf(iso ? v : -(t * t), p.hg_a2, p.hg_c0);
+        const float sint = sqrt_spin(fmaf(-cost, cost, 1.0f));
         p.dx = sint * cosp;
         p.dy = sint * sinp;
         p.dz = copysignf(cost, p.dz * cost);
     } else {
         /* one division serves the HG inversion (1/D = b r) and the rotation (1/b = D r) */
         const float r = rcp_rn(D * b);
-        float cost = fmaf(-2.0f, v, 3.0f);
-        if (!iso) {
-            const float br = b * r; /* 1 / D */
-            cost = fmaf(-(br * br), p.hg_a2, p.hg_c0);
-        }
+        const float br = b * r; /* 1 / D */
+        const float cost = fmaf(iso ? v : -(br * br), p.hg_a2, p.hg_c0);
         const float ab = fmaf(-cost, cost, 1.0f) * b;
-        const float P = sqrt_rn(ab);
+        const float P = sqrt_spin(ab);
         const float Q = P * (D * r);
         /* ux' = ux (cos + uz Q cos psi) - uy Q sin psi, uy' = uy (cos + uz Q cos psi) +
          * ux Q sin psi: the bracket is shared */
@@ -331,21 +339,25 @@ template <bool REPLAY>
 __device__ __forceinline__ bool hop(Photon& p, float& s)
 {
     s = mcml_neglog_u32(mcml_mwc_next(&p.rng), LG_TAB) * p.mutr;
-    const bool lo = fmaf(s, p.dz, p.z) < p.zmin;
-    if (lo) s = (p.zmin - p.z) / p.dz;
-    const bool hi = fmaf(s, p.dz, p.z) > p.zmax;
-    if (hi) s = (p.zmax - p.z) / p.dz;
+    /* the uncut landing passes at most one of the two (zmin <= zmax): one branch for both cuts */
+    const float zn = fmaf(s, p.dz, p.z);
+    const bool lo = zn < p.zmin;
+    bool hi = zn > p.zmax;
+    if (lo || hi) {
+        s = ((lo ? p.zmin : p.zmax) - p.z) / p.dz;
+        /* a hop cut at zmin is tested against zmax again (transport.cu:387), which only a
+         * layer thinner than the rounding of the cut can fail */
+        if (lo && fmaf(s, p.dz, p.z) > p.zmax) {
+            hi = true;
+            s = (p.zmax - p.z) / p.dz;
+        }
+    }
     p.x = fmaf(p.dx, s, p.x);
     p.y = fmaf(p.dy, s, p.y);
-    p.z = fmaf(p.dz, s, p.z);
+    /* transport.cu:394-395 as one median: an uncut step already lies in [zmin, zmax] */
+    p.z = __builtin_amdgcn_fmed3f(fmaf(p.dz, s, p.z), p.zmin, p.zmax);
     p.steps++;
     if (REPLAY) p.pl_cur += s; /* transport.cu:529: pathlength of the layer the hop started in */
-    /* transport.cu:394-395.  An uncut step lands on the value both tests saw, which lies in
-     * [zmin, zmax], so only a step cut at a boundary can need the clamp. */
-    if (lo || hi) {
-        if (p.z > p.zmax) p.z = p.zmax;
-        if (p.z < p.zmin) p.z = p.zmin;
-    }
     return lo != hi;
 }
 

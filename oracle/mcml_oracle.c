@@ -64,6 +64,7 @@ int oracle_derive(const mcml_sim* s, mcml_derived* d)
     float zmin = 0.0f, zmax = 0.0f;
     for (int l = 1; l <= L; l++) { /* io.cpp:210-221 */
         float h = (l < L) ? s->layers[l - 1].thickness : 0.0f; /* last thickness unread (io.cpp:146) */
+        if (!(h >= 0.0f)) return MCML_ERR_INPUT; /* as mcml_derive */
         zmax += h;
         d->layer[l].n = s->layers[l - 1].n;
         d->layer[l].mua = s->layers[l - 1].mua;
@@ -169,6 +170,10 @@ void oracle_launch(const mcml_derived* d, uint64_t seed, uint32_t set, uint64_t 
  *                                                           |d| stays at a few ulp, so
  *                                                           3d^2/8 is below 1e-13)
  * Returns cos(theta) for the tests. */
+/* The spin's square roots: sqrtf(x) for x > 0, else 0 (sin(theta) = 0 where cos(theta)
+ * rounded to or past +-1). */
+static float spin_sqrt(float x) { return x > 0.0f ? sqrtf(x) : 0.0f; }
+
 float oracle_spin(oracle_photon* p, float g)
 {
     /* u = 2 - v with v = 1.k in [1, 2) (mcml_uniform): the HG denominator 1 - g + 2gu is
@@ -191,7 +196,7 @@ float oracle_spin(oracle_photon* p, float g)
             float t = 1.0f / D;
             cost = fmaf(-(t * t), a2, c0);
         }
-        float sint = sqrtf(fmaxf(fmaf(-cost, cost, 1.0f), 0.0f)); /* cos(theta) may round past +-1 */
+        float sint = spin_sqrt(fmaf(-cost, cost, 1.0f)); /* cos(theta) may round past +-1 */
         p->dx = sint * cosp;
         p->dy = sint * sinp;
         p->dz = copysignf(cost, p->dz * cost);
@@ -204,7 +209,7 @@ float oracle_spin(oracle_photon* p, float g)
             cost = fmaf(-(br * br), a2, c0);
         }
         float ab = fmaf(-cost, cost, 1.0f) * b;
-        float P = sqrtf(fmaxf(ab, 0.0f)); /* cos(theta) may round past +-1: sin(theta) = 0 */
+        float P = spin_sqrt(ab); /* cos(theta) may round past +-1: sin(theta) = 0 */
         float Q = P * (D * r);
         /* MCML's update (Wang et al. 1995) with the shared bracket cos + uz Q cos(psi):
          * ux' = ux (cos + uz Q cos psi) - uy Q sin psi, uy' = uy (...) + ux Q sin psi */

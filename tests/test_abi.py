@@ -70,3 +70,11 @@ def test_invalid_arguments_rejected_before_device_use():
     assert lib.mcml_derive(C.byref(sim), C.byref(d)) == abi.ERR_LIMIT
     sim.number_layers, sim.num_sds = 2, 31
     assert lib.mcml_derive(C.byref(sim), C.byref(d)) == abi.ERR_LIMIT
+    # a negative or NaN thickness (the reference reads it unchecked) would put z_min above z_max
+    sim.number_layers, sim.num_sds = 3, 1
+    sim.layers[0].thickness, sim.layers[1].thickness = 0.1, -0.05
+    assert lib.mcml_derive(C.byref(sim), C.byref(d)) == abi.ERR_INPUT
+    sim.layers[1].thickness = float("nan")
+    assert lib.mcml_derive(C.byref(sim), C.byref(d)) == abi.ERR_INPUT
+    sim.layers[1].thickness = 0.0  # a layer of zero thickness is accepted
+    assert lib.mcml_derive(C.byref(sim), C.byref(d)) == 0
