@@ -11,8 +11,8 @@ mcml_lg_table[j] = {c_j, L_j}, j = 0..127, with
     the table of mcml_neglog_u32, after Tang, "Table-driven implementation of the
     logarithm function in IEEE floating-point arithmetic", ACM TOMS 16(4), 1990: for a
     mantissa m in cell j, r = m c_j - 1 is small (0 <= r < 1/128 up to rounding) and
-    ln m = L_j + ln(1 + r).  Cell 0 has c_0 = 1 and L_0 = 0, so u = 1 gives -ln u = 0
-    exactly.
+    ln m = L_j + ln(1 + r).  Cell 0 has c_0 = 1 and L_0 = 0, so a mantissa of exactly 1
+    gives ln m = 0 exactly.
 
 Every entry is the double-precision value rounded to nearest binary32, so the table is a
 function of the formulas alone; tests/test_detmath.py recomputes it and compares bits.

@@ -119,27 +119,27 @@ MCML_HD uint32_t mcml_exp_magic(uint32_t b)
 #endif
 }
 
-/* -ln(u) for the uniform u = n 2^-24, n = (x >> 8) + 1 in [1, 2^24], from one 32-bit draw x:
- * the exponential step of the photon walk.  Table-driven (Tang, ACM TOMS 16(4), 1990):
- * float(n) = 2^e m is exact, m in [1, 2) lies in cell j of 128 (its top seven fraction
- * bits); with the tabulated c_j ~ 1 / (1 + j/128) and L_j = -ln(c_j) (mcml_tables.h, `tab` =
- * mcml_lg_table), r = m c_j - 1 (one fma, 0 <= r < 1/128 up to rounding) and
- * ln u = (e - 24) ln 2 + L_j + ln(1 + r),  ln(1 + r) = r (1 + r (r/3 - 1/2))  (error < r^4/4
- * < 1e-9), evaluated as three fmas ending on L_j.  (e - 24) ln 2 is one fma with ln 2
- * rounded to binary32: |e - 24| <= 24, so the constant adds at most 4.6e-8 where -ln u is
- * near 16.6 and 1.9e-9 below 1.  Over all 2^24 inputs the error is below 6.3e-8 absolute
- * where -ln u < 1 and 8.7e-8 relative above (tests/test_detmath.py checks every n), and u = 1
- * (cell 0: c_0 = 1, L_0 = 0, r = 0) gives exactly 0.  No input gives a positive logarithm
- * (every one of the 2^24 is checked), so a step is never negative without a clamp. */
+/* -ln(u), the exponential step of the photon walk, for the uniform u = (2k + 1) 2^-24 drawn
+ * from the low 23 bits k of one 32-bit draw x: the midpoint of one of 2^23 equal cells of
+ * (0, 1), as curand_uniform's (x + 1/2) 2^-32 is at 32 bits, formed exactly as
+ * 1.k - (1 - 2^-24).  So 2^-24 <= u <= 1 - 2^-24 and every step is positive.  Table-driven
+ * (Tang, ACM TOMS 16(4), 1990): u = 2^e m with m in [1, 2) in cell j of 128 (its top seven
+ * fraction bits); with the tabulated c_j ~ 1 / (1 + j/128) and L_j = -ln(c_j) (mcml_tables.h,
+ * `tab` = mcml_lg_table), r = m c_j - 1 (one fma, 0 <= r < 1/128 up to rounding) and
+ * ln u = e ln 2 + L_j + ln(1 + r),  ln(1 + r) = r (1 + r (r/3 - 1/2))  (error < r^4/4 < 1e-9),
+ * evaluated as three fmas ending on L_j.  e ln 2 is one fma with ln 2 rounded to binary32:
+ * -24 <= e <= -1, so the constant adds at most 4.6e-8 where -ln u is near 16.6.  Over all
+ * 2^23 inputs the error is below 6.3e-8 absolute where -ln u < 1 and 8.7e-8 relative above,
+ * and the smallest step is 5.96e-8 mean free paths (tests/test_detmath.py checks every k). */
 MCML_HD float mcml_neglog_u32(uint32_t x, const mcml_pair* tab)
 {
-    const uint32_t n = (x >> 8) + 1u;
-    const uint32_t b = mcml_f2u((float)n);
+    const float u = mcml_one_plus(x) - 0x1.fffffep-1f; /* exact */
+    const uint32_t b = mcml_f2u(u);
     const uint32_t j = (b >> 16) & 127u;
     const float m = mcml_u2f((b & 0x007FFFFFu) | 0x3F800000u);
-    /* e - 24 = (biased exponent) - 151, exactly: 2^23 + biased exponent as a float (its bits
-     * 0x4B000000 | b >> 23, one funnel shift on the GPU) minus 2^23 + 151 */
-    const float le = mcml_u2f(mcml_exp_magic(b)) - 8388759.0f;
+    /* e = (biased exponent) - 127, exactly: 2^23 + biased exponent as a float (its bits
+     * 0x4B000000 | b >> 23, one funnel shift on the GPU) minus 2^23 + 127 */
+    const float le = mcml_u2f(mcml_exp_magic(b)) - 8388735.0f;
     const mcml_pair t = tab[j];
     const float c = t.a, L = t.b;
     const float r = fmaf(m, c, -1.0f);

@@ -381,6 +381,17 @@ __device__ __forceinline__ int survive(Photon& p)
     return ST_DEAD;
 }
 
+/* survive() for a photon at the step cap or at or below WEIGHTI: true = it ends. */
+__device__ __forceinline__ bool survive_ends(Photon& p)
+{
+    if (p.steps >= MCML_MAX_SCATTER || p.w == 0.0f) return true;
+    if (mcml_uniform(&p.rng) < MCML_CHANCE) {
+        p.w = rintf(p.w / MCML_CHANCE);
+        return false;
+    }
+    return true;
+}
+
 /* Absorbance bookkeeping of one replayed photon (transport.cu:580-600): the first drop goes
  * to A0_z, later ones to A_rz, merged while consecutive drops stay in one cell.
  *
@@ -435,41 +446,41 @@ __device__ __forceinline__ void flush_run(GridLane& gl, int nr)
 }
 
 /* Interaction (transport.cu:430-437 / 573-603): drop weight and scatter when the hop did
- * not end on a boundary and s > 0, then the survival test. */
+ * not end on a boundary, then the survival test.  The reference tests s > 0, with s set to
+ * 0 on a boundary; an uncut hop's s = -ln(u) mutr is at least 5.96e-8 mutr (u < 1), positive
+ * for any layer with mua + mus below 4e37 /cm, so "not on a boundary" is the same test. */
 template <bool REPLAY>
-__device__ __forceinline__ int interact(Photon& p, float s, GridLane& gl, const ReplayArgs* ra, uint32_t& n_int)
+__device__ __forceinline__ int interact(Photon& p, GridLane& gl, const ReplayArgs* ra, uint32_t& n_int)
 {
-    if (s > 0.0f) {
-        n_int++;
-        const float w_temp = rintf(p.albedo * p.w);
-        p.w -= w_temp;
-        if (REPLAY && gl.a_rz != nullptr) {
-            float qz = p.z * ra->rdz;
-            qz = fminf(fmaxf(qz, 0.0f), (float)(ra->nz - 1));
-            const int zb = (int)qz;
-            if (p.first) {
-                if (zb < TILE_Z && gl.grp == gl.tile_grp)
-                    atomicAdd(&gl.tile[TILE_Z * TILE_R + zb], (unsigned long long)w_temp);
-                else if (zb < TILE_Z && gl.stage != nullptr)
-                    atomicAdd(&gl.stage[TILE_Z * TILE_R + zb], (unsigned long long)w_temp);
-                else
-                    atomicAdd(&gl.a0_z[zb], (unsigned long long)w_temp);
+    n_int++;
+    const float w_temp = rintf(p.albedo * p.w);
+    p.w -= w_temp;
+    if (REPLAY && gl.a_rz != nullptr) {
+        float qz = p.z * ra->rdz;
+        qz = fminf(fmaxf(qz, 0.0f), (float)(ra->nz - 1));
+        const int zb = (int)qz;
+        if (p.first) {
+            if (zb < TILE_Z && gl.grp == gl.tile_grp)
+                atomicAdd(&gl.tile[TILE_Z * TILE_R + zb], (unsigned long long)w_temp);
+            else if (zb < TILE_Z && gl.stage != nullptr)
+                atomicAdd(&gl.stage[TILE_Z * TILE_R + zb], (unsigned long long)w_temp);
+            else
+                atomicAdd(&gl.a0_z[zb], (unsigned long long)w_temp);
+        } else {
+            float qr = sqrt_rn(p.x * p.x + p.y * p.y) * ra->rdr;
+            qr = fminf(fmaxf(qr, 0.0f), (float)(ra->nr - 1));
+            const uint64_t key = ((uint64_t)(uint32_t)zb << 32) | (uint32_t)(int)qr;
+            if (key == gl.rl_key) {
+                gl.rl_w += (uint64_t)w_temp;
             } else {
-                float qr = sqrt_rn(p.x * p.x + p.y * p.y) * ra->rdr;
-                qr = fminf(fmaxf(qr, 0.0f), (float)(ra->nr - 1));
-                const uint64_t key = ((uint64_t)(uint32_t)zb << 32) | (uint32_t)(int)qr;
-                if (key == gl.rl_key) {
-                    gl.rl_w += (uint64_t)w_temp;
-                } else {
-                    flush_run(gl, ra->nr);
-                    gl.rl_key = key;
-                    gl.rl_w = (uint64_t)w_temp;
-                }
+                flush_run(gl, ra->nr);
+                gl.rl_key = key;
+                gl.rl_w = (uint64_t)w_temp;
             }
         }
-        p.first = 0;
-        spin(p);
     }
+    p.first = 0;
+    spin(p);
     return survive(p);
 }
 
@@ -863,7 +874,7 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
             bool bnd = false;
             if (act) bnd = hop<PL>(p, s);
             if (act && bnd) pending = true; /* the layer beyond follows from dz (beyond_layer) */
-            if (act && !bnd && s > 0.0f) {
+            if (act && !bnd) { /* interact<false>: s > 0 on every uncut hop */
                 const float w_temp = rintf(p.albedo * p.w);
                 p.w -= w_temp;
                 if (GRIDS && rp_mask != 0) {
@@ -875,7 +886,9 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                 }
                 spin(p);
             }
-            if (act && !bnd) st = survive(p);
+            /* survive() only where its first tests can fail: one branch on the common path */
+            if (act && !bnd && (p.steps >= MCML_MAX_SCATTER || !(p.w > (float)MCML_WEIGHTI)))
+                st = survive_ends(p) ? ST_DEAD : ST_ALIVE;
         }
         if (resolve_now(pending, alive, bnd_min) && pending) {
             pending = false;
@@ -1076,7 +1089,7 @@ extern "C" __global__ void __launch_bounds__(256) mcml_refshape_pass(RefShapeArg
                 c_bnd++;
                 st = cross<false>(p, beyond_layer(p), S, 0, mask, nullptr, 0);
             }
-            if (!bnd && s > 0.0f) {
+            if (!bnd) { /* interact<false>: s > 0 on every uncut hop */
                 n_int++;
                 p.w -= rintf(p.albedo * p.w);
                 spin(p);
@@ -1287,7 +1300,7 @@ extern "C" __global__ void __launch_bounds__(REPLAY_BLOCK) mcml_replay(ReplayArg
             bool bnd = false;
             if (act) bnd = hop<true>(p, s);
             if (act && bnd) pending = true;
-            if (act && !bnd) st = interact<true>(p, s, gl, &a, n_int);
+            if (act && !bnd) st = interact<true>(p, gl, &a, n_int);
         }
         if (resolve_now(pending, alive, bnd_min) && pending) {
             pending = false;

@@ -255,7 +255,12 @@ int oracle_reflect(const mcml_derived* d, oracle_photon* p, int new_layer)
         return 0;
     }
     float e = (n1 * n1) / (n2 * n2) * (1.0f - ci * ci);
-    r = 2.0f * sqrtf((1.0f - ci * ci) * (1.0f - e) * e * ci * ci);
+    /* Just inside the critical angle the test above can round to "not total" while e, the
+     * squared sine of the refracted angle, rounds above 1: the reference's sqrtf of the
+     * negative arguments here and below is then NaN (transport.cu:831, 848) and the photon
+     * is lost as NaN.  Taking them as 0 gives reflectance 1 to rounding (sin = 1, total
+     * reflection) and a grazing direction otherwise. */
+    r = 2.0f * sqrtf(fmaxf((1.0f - ci * ci) * (1.0f - e) * e * ci * ci, 0.0f));
     e = e + (ci * ci) * (1.0f - 2.0f * e);
     r = e * ((1.0f - e - r) / ((1.0f - e + r) * (e + r)));
     if (urand(p) <= r) {
@@ -266,7 +271,7 @@ int oracle_reflect(const mcml_derived* d, oracle_photon* p, int new_layer)
     e = r * r * (1.0f - ci * ci);
     p->dx = p->dx * r;
     p->dy = p->dy * r;
-    p->dz = copysignf(sqrtf(1.0f - e), p->dz);
+    p->dz = copysignf(sqrtf(fmaxf(1.0f - e, 0.0f)), p->dz);
     p->layer = new_layer;
     return 0;
 }

@@ -43,21 +43,22 @@ def neglog(x):
 
 
 def test_neglog_every_step_value():
-    """-ln(u) for all 2^24 values u = n 2^-24 the step can draw (x = (n - 1) << 8): within
-    7.5e-8 + 9e-8 |ln u| of float64 (6.3e-8 absolute below 1, 8.7e-8 relative above),
-    never negative, exactly 0 at u = 1, and the largest step is -ln(2^-24) = 16.64 mean
-    free paths."""
-    n = np.arange(1, 2 ** 24 + 1, dtype=np.uint64)
-    x = ((n - 1) << 8).astype(np.uint32)
+    """-ln(u) for all 2^23 values u = (2k + 1) 2^-24 the step can draw (k = the draw's low
+    23 bits): within 7.5e-8 + 9e-8 |ln u| of float64 (6.3e-8 absolute below 1, 8.7e-8
+    relative above), positive everywhere (no zero-length step), smallest at k = 2^23 - 1
+    (-ln(1 - 2^-24) = 5.96e-8) and largest at k = 0 (-ln(2^-24) = 16.64 mean free paths)."""
+    k = np.arange(2 ** 23, dtype=np.uint64)
+    x = k.astype(np.uint32)
     y = neglog(x)
-    ref = -np.log(n.astype(np.float64) * 2.0 ** -24)
+    ref = -np.log((2 * k + 1).astype(np.float64) * 2.0 ** -24)
     err = np.abs(y.astype(np.float64) - ref)
     assert (err <= 7.5e-8 + 9e-8 * ref).all(), err.max()
-    assert (y >= 0).all() and y[-1] == 0.0  # never negative: the kernel does not clamp
+    assert (y > 0).all()
+    assert y.argmin() == len(y) - 1 and abs(y[-1] - 2.0 ** -24) < 1e-14
     assert abs(y[0] - 24 * np.log(2)) < 2e-6
-    # the low 8 bits of the draw are not used
-    lo = neglog((x[::4097] | np.uint32(0xA5)))
-    assert np.array_equal(lo.view(np.uint32), y[::4097].view(np.uint32))
+    # the high 9 bits of the draw are not used
+    hi = neglog(x[::4097] | np.uint32(0xA5800000))
+    assert np.array_equal(hi.view(np.uint32), y[::4097].view(np.uint32))
 
 
 def test_sincos_u32():

@@ -110,6 +110,33 @@ def test_fresnel_against_textbook(n1, n2, deg):
         assert dout[2] < 0
 
 
+def test_critical_angle_rounding_reflects_instead_of_nan():
+    """Just inside the critical angle of 1.4 -> 1.38 there are 12 floats cos(theta_i) where
+    the total-reflection test (n2^2 < n1^2 (1 - ci^2), in float) fails while the squared
+    sine of the refracted angle, (n1^2 / n2^2)(1 - ci^2), rounds above 1.  The reference's
+    Fresnel square roots are NaN there (transport.cu:831, 848); here they are 0, so the
+    photon reflects (reflectance 1 to rounding) and never carries a NaN direction.  The
+    kernel's square root clamps negative arguments alike."""
+    f = np.float32
+    n1, n2 = f(1.4), f(1.38)
+    d = _two_layer(1.4, 1.38)
+    base = np.sqrt(f(1) - (n2 / n1) ** 2).astype(np.float32).view(np.uint32)
+    borderline = []
+    for k in range(-200, 200):
+        ci = np.uint32(int(base) + k).view(np.float32)
+        s = f(1) - ci * ci
+        if not (n2 * n2 < n1 * n1 * s) and (n1 * n1) / (n2 * n2) * s > 1:
+            borderline.append(ci)
+    assert len(borderline) == 12
+    for ci in borderline:
+        si = np.sqrt(f(1) - ci * ci)
+        direction = np.array([si, 0.0, -ci], np.float32)
+        dout = np.zeros(3, np.float32)
+        frac = O.lib().oracle_reflect_fraction(C_derived(d), 1, 0, direction, 5, 20000, dout)
+        assert frac > 0.999
+        assert np.isfinite(dout).all()
+
+
 def test_matched_interface_transmits_without_turning():
     d = _two_layer(1.4, 1.4)
     direction = np.array([0.6, 0.0, -0.8], np.float32)

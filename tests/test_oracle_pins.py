@@ -107,7 +107,7 @@ def _mwc_first_draw(seed, k):
 
 def test_pencil_beam_first_drop_depth():
     """A normal-incidence pencil beam into an index-matched medium hops straight down by
-    s1 = -ln(u) / (mua + mus), u = ((x1 >> 8) + 1) 2^-24 from the photon's first draw x1,
+    s1 = -ln(u) / (mua + mus), u = (2 (x1 & 0x7FFFFF) + 1) 2^-24 from the photon's first draw x1,
     and books its first drop there: A0_z[floor(s1 / dz)] holds mua / (mua + mus) of the
     start weight and no other A0_z cell is touched."""
     mua, mus, dz = 1.0, 99.0, 0.001
@@ -120,7 +120,7 @@ def test_pencil_beam_first_drop_depth():
     checked = 0
     for k in range(300):
         x1 = _mwc_first_draw(5, k)
-        s1 = -np.log(((x1 >> 8) + 1) / 2.0 ** 24) / (mua + mus)
+        s1 = -np.log((2 * (x1 & 0x7FFFFF) + 1) / 2.0 ** 24) / (mua + mus)
         q = s1 / dz
         if abs(q - round(q)) < 1e-4:
             continue
