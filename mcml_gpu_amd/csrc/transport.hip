@@ -208,14 +208,8 @@ __device__ __forceinline__ void spin(Photon& p)
     const float b = fmaf(-p.dz, p.dz, 1.0f);
     /* cos(theta) = fma(X, hg_a2, hg_c0): X = -1/D^2, or for an isotropic layer X = v with
      * hg_a2 = -2, hg_c0 = 3 (3 - 2v, one rounding) */
-    if (b == 0.0f) {
-        const float t = 1.0f / D;
-        const float cost = fmaf(iso ? v : -(t * t), p.hg_a2, p.hg_c0);
-        const float sint = sqrt_spin(fmaf(-cost, cost, 1.0f));
-        p.dx = sint * cosp;
-        p.dy = sint * sinp;
-        p.dz = copysignf(cost, p.dz * cost);
-    } else {
+    const float dz0 = p.dz;
+    {
         /* one division serves the HG inversion (1/D = b r) and the rotation (1/b = D r) */
         const float r = rcp_rn(D * b);
         const float br = b * r; /* 1 / D */
@@ -232,6 +226,14 @@ __device__ __forceinline__ void spin(Photon& p)
         p.dz = fmaf(-P, cosp, p.dz * cost);
         p.dx = cxp;
         p.dy = cyp;
+    }
+    if (b == 0.0f) { /* travelling along z: the lines above divided by 0, replace their result */
+        const float t = 1.0f / D;
+        const float cost = fmaf(iso ? v : -(t * t), p.hg_a2, p.hg_c0);
+        const float sint = sqrt_spin(fmaf(-cost, cost, 1.0f));
+        p.dx = sint * cosp;
+        p.dy = sint * sinp;
+        p.dz = copysignf(cost, dz0 * cost);
     }
     const float d = fmaf(p.dx, p.dx, fmaf(p.dy, p.dy, fmaf(p.dz, p.dz, -1.0f)));
     const float r = fmaf(d, -0.5f, 1.0f);
@@ -860,7 +862,7 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
             s_refill += (!alive && !q.dry) ? 1u : 0u;
             s_dry += (!alive && q.dry) ? 1u : 0u;
         }
-        int st = ST_ALIVE;
+        bool ended = false; /* weight or step cap, a lane mask */
         uint32_t mask = 0;
         /* several steps between wavefront-level decisions: a lane that reaches a boundary
          * or ends its photon sits out the rest of the block */
@@ -869,7 +871,7 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
             /* interact<false> written as flat conditions: one level of divergence around
              * the spin instead of three, so the compiler keeps the direction in place
              * rather than copying it in and out of each level */
-            const bool act = alive && !pending && st == ST_ALIVE;
+            const bool act = alive && !pending && !ended;
             float s = 0.0f;
             bool bnd = false;
             if (act) bnd = hop<PL>(p, s);
@@ -885,11 +887,11 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                     n_int++;
                 }
                 spin(p);
+                /* survive() only where its first tests can fail: one branch on the common path */
+                if (p.steps >= MCML_MAX_SCATTER || !(p.w > (float)MCML_WEIGHTI)) ended = survive_ends(p);
             }
-            /* survive() only where its first tests can fail: one branch on the common path */
-            if (act && !bnd && (p.steps >= MCML_MAX_SCATTER || !(p.w > (float)MCML_WEIGHTI)))
-                st = survive_ends(p) ? ST_DEAD : ST_ALIVE;
         }
+        int st = ended ? ST_DEAD : ST_ALIVE;
         if (resolve_now(pending, alive, bnd_min) && pending) {
             pending = false;
             if (!GRIDS || rp_mask == 0) c_bnd++;
