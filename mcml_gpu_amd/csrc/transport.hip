@@ -736,9 +736,10 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
     uint64_t clo = 0, chi = 0;
     /* per-lane work counters (statistics only; the results do not depend on them).  A
      * lane's steps run one after another, about a microsecond each at full occupancy, so
-     * its totals stay below 2^32 unless one launch runs for more than an hour. */
-    uint32_t c_photons = 0, c_steps = 0, c_bnd = 0, c_int = 0;
-    uint32_t n_int = 0;
+     * its totals stay below 2^32 unless one launch runs for more than an hour.  A hop
+     * either ends on a boundary or interacts (its step is never 0), so interactions are
+     * counted as steps - boundary hops. */
+    uint32_t c_photons = 0, c_steps = 0, c_bnd = 0;
     const int refill_min = a.refill_min < 1 ? 1 : a.refill_min;
     const int bnd_min = a.bnd_min < 1 ? 1 : a.bnd_min;
     uint32_t s_step = 0, s_pend = 0, s_refill = 0, s_dry = 0; /* lane-blocks by state (STATS) */
@@ -849,9 +850,7 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                     col[SP_RL_W * cs] = __uint_as_float(rl_w);
                     if (PL)
                         for (int l = 0; l < L; l++) col[(SP_PL0 + l) * cs] = pl_lane[l * pl_stride];
-                    c_int += n_int; /* the photon's interactions so far; its steps count where it ends */
-                    n_int = 0;
-                    alive = false;
+                    alive = false; /* its steps count where it ends */
                 }
                 break;
             }
@@ -883,8 +882,6 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                     book_drop(a, p, set * (uint32_t)sets[set].num_sds + (uint32_t)(__ffs(rp_mask) - 2), w_temp,
                               rl_key, rl_w);
                     p.first = 0;
-                } else {
-                    n_int++;
                 }
                 spin(p);
                 /* survive() only where its first tests can fail: one branch on the common path */
@@ -909,8 +906,6 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
             alive = false;
             if (PL) pl_lane[(p.pl_layer - 1) * pl_stride] = p.pl_cur;
             c_steps += (uint32_t)p.steps;
-            c_int += n_int;
-            n_int = 0;
             if (SUMS && st == ST_DETECTED) {
                 const DevSet* S = sets + set;
                 const int ns = S->num_sds;
@@ -947,7 +942,10 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
             alive = true;
         }
     }
-    const uint64_t s0 = wave_sum(c_photons), s1 = wave_sum(c_steps), s2 = wave_sum(c_bnd), s3 = wave_sum(c_int);
+    /* A photon's boundary hops count in the launch that resolves them and its steps in the
+     * launch that ends it, so one launch's steps - boundary hops can be negative: the
+     * counters add modulo 2^64, and their sums over a batch's launches are exact. */
+    const uint64_t s0 = wave_sum(c_photons), s1 = wave_sum(c_steps), s2 = wave_sum(c_bnd), s3 = s1 - s2;
     if (lane_id() == 0) { /* wave_sum widens to 64 bits before adding */
         atomicAdd(&a.counters[CNT_PHOTONS], (unsigned long long)s0);
         atomicAdd(&a.counters[CNT_STEPS], (unsigned long long)s1);
