@@ -178,6 +178,19 @@ __device__ __forceinline__ float sqrt_rn(float x)
     return fmaf(r, h, s);
 }
 
+/* a / b from the correctly rounded reciprocal y = rcp_rn(b): q = a y, then q + (a - b q) y
+ * with the residual exact by fma.  By Markstein's theorem (y within half an ulp of 1/b, q
+ * within an ulp of a/b) this is the IEEE quotient unless it over- or underflows; the oracle
+ * forms the hop's cut distance the same way, so the two agree in every case.  6 instructions
+ * (one transcendental) against 11 for the IEEE division sequence. */
+__device__ __forceinline__ float div_rn(float a, float b)
+{
+    const float y = rcp_rn(b);
+    const float q = a * y;
+    const float r = fmaf(-b, q, a);
+    return fmaf(r, y, q);
+}
+
 /* sqrt_rn for the spin's arguments, (1 - cos^2) and (1 - cos^2)(1 - dz^2).  Each factor is
  * 1 - c^2 formed by one fma from a float c: 0 at |c| = 1 and otherwise at least 2^-24 in
  * magnitude (the floats next to 1 are 1 - 2^-24 and 1 + 2^-23), so the argument is 0 or at
@@ -346,12 +359,12 @@ __device__ __forceinline__ bool hop(Photon& p, float& s)
     const bool lo = zn < p.zmin;
     bool hi = zn > p.zmax;
     if (lo || hi) {
-        s = ((lo ? p.zmin : p.zmax) - p.z) / p.dz;
+        s = div_rn((lo ? p.zmin : p.zmax) - p.z, p.dz);
         /* a hop cut at zmin is tested against zmax again (transport.cu:387), which only a
          * layer thinner than the rounding of the cut can fail */
         if (lo && fmaf(s, p.dz, p.z) > p.zmax) {
             hi = true;
-            s = (p.zmax - p.z) / p.dz;
+            s = div_rn(p.zmax - p.z, p.dz);
         }
     }
     p.x = fmaf(p.dx, s, p.x);
