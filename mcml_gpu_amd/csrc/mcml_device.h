@@ -76,6 +76,8 @@ struct FirstPassArgs {
     int32_t bnd_min;           /* resolve boundary hits once this many lanes wait on one */
     int32_t lds_sets;          /* 1: stage the set table in LDS (dynamic shared memory) */
     int32_t num_layers;        /* L (pathlength recording) */
+    int32_t any_iso;           /* 1: some tissue layer of some set has g = 0 (isotropic) */
+    int32_t pad_iso;
     uint64_t total;            /* total tickets */
     uint64_t seed;
     const uint32_t* mult;      /* MWC multiplier table */

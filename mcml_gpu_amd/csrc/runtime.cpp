@@ -282,6 +282,15 @@ double now_ms()
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+/* 1 when a tissue layer of set S scatters isotropically (g = 0): the transport then keeps
+ * the isotropic cos(theta) select in its step. */
+int devset_iso(const DevSet& S, int num_layers)
+{
+    for (int l = 1; l <= num_layers; l++)
+        if (S.layer[l].hg_cn == 0.0f) return 1;
+    return 0;
+}
+
 void fill_devset(const mcml_derived& d, DevSet& S)
 {
     memset(&S, 0, sizeof(S));
@@ -692,6 +701,7 @@ int first_pass(mcml_ctx* c, const std::vector<mcml_derived>& der, const mcml_opt
     /* a small set table is staged in LDS by every workgroup; a large batch reads it from
      * global memory (it is only read at launch, layer changes and boundary hits) */
     a.lds_sets = sizeof(DevSet) * (size_t)nsets <= kLdsTableMax ? 1 : 0;
+    for (int j = 0; j < nsets && !a.any_iso; j++) a.any_iso = devset_iso(hs[(size_t)j], der[0].num_layers);
     int occ = 0;
     HIP_TRY(c, mcml_occupancy_first_pass(a, block, &occ));
     if (occ < 1) occ = 1;
@@ -1474,6 +1484,7 @@ extern "C" int mcml_run_batch_sums_noalloc(mcml_ctx* c, const mcml_sim* sims, in
     a.num_layers = L;
     a.row_stride = 1;
     a.lds_sets = sizeof(DevSet) * (size_t)nsets <= kLdsTableMax ? 1 : 0;
+    for (int j = 0; j < nsets && !a.any_iso; j++) a.any_iso = devset_iso(hs[j], L);
     int occ = 0;
     HIP_TRY(c, mcml_occupancy_first_pass_sums(a, block, &occ));
     if (occ < 1) occ = 1;

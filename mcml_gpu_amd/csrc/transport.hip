@@ -209,12 +209,13 @@ __device__ __forceinline__ float sqrt_spin(float x)
  * P = sin(theta) sqrt(1-dz^2), Q = sin(theta) / sqrt(1-dz^2), and a first-order
  * rsqrt about 1 for the renormalisation (the norm never drifts more than a few ulp, so the
  * second-order term 3d^2/8 is below 1e-13, far under half an ulp of 1). */
+template <bool ISO>
 __device__ __forceinline__ void spin(Photon& p)
 {
     /* the HG uniform u = 2 - v, v = 1.k in [1, 2): D = 1 - g + 2gu = (1 + 3g) - 2gv, and the
      * isotropic cos(theta) = 2u - 1 = 3 - 2v exactly */
     const float v = mcml_one_plus(mcml_mwc_next(&p.rng));
-    const bool iso = p.hg_cn == 0.0f; /* g == 0 */
+    const bool iso = ISO && p.hg_cn == 0.0f; /* g == 0 (ISO false: no layer of the launch has g = 0) */
     const float D = fmaf(p.hg_cn, v, p.hg_e); /* exactly 1 for g = 0 */
     float sinp, cosp;
     mcml_sincos_u32(mcml_mwc_next(&p.rng), SC_TAB, &sinp, &cosp);
@@ -495,7 +496,7 @@ __device__ __forceinline__ int interact(Photon& p, GridLane& gl, const ReplayArg
         }
     }
     p.first = 0;
-    spin(p);
+    spin<true>(p);
     return survive(p);
 }
 
@@ -723,7 +724,7 @@ __device__ __forceinline__ void book_drop(const FirstPassArgs& a, const Photon& 
 /* LDS_SETS: the (small) set table is staged in LDS and read there with LDS instructions;
  * otherwise it is read from global memory (a large batch).  Both are compiled into each
  * kernel and chosen by a.lds_sets, so that no access goes through flat addressing. */
-template <bool PL, bool SUMS, bool STATS, bool GRIDS, bool LDS_SETS>
+template <bool PL, bool SUMS, bool STATS, bool GRIDS, bool LDS_SETS, bool ISO>
 __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned char* lds_raw)
 {
     const DevSet* sets = a.sets;
@@ -896,7 +897,7 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
                               rl_key, rl_w);
                     p.first = 0;
                 }
-                spin(p);
+                spin<ISO>(p);
                 /* survive() only where its first tests can fail: one branch on the common path */
                 if (p.steps >= MCML_MAX_SCATTER || !(p.w > (float)MCML_WEIGHTI)) ended = survive_ends(p);
             }
@@ -980,10 +981,15 @@ __device__ __forceinline__ void first_pass_body(const FirstPassArgs& a, unsigned
 template <bool PL, bool SUMS, bool STATS = false, bool GRIDS = false>
 __device__ __forceinline__ void first_pass_entry(const FirstPassArgs& a, unsigned char* lds_raw)
 {
-    if (a.lds_sets)
-        first_pass_body<PL, SUMS, STATS, GRIDS, true>(a, lds_raw);
+    /* wave-uniform: one body per launch, so each launch's code is that of one body */
+    if (a.lds_sets && a.any_iso)
+        first_pass_body<PL, SUMS, STATS, GRIDS, true, true>(a, lds_raw);
+    else if (a.lds_sets)
+        first_pass_body<PL, SUMS, STATS, GRIDS, true, false>(a, lds_raw);
+    else if (a.any_iso)
+        first_pass_body<PL, SUMS, STATS, GRIDS, false, true>(a, lds_raw);
     else
-        first_pass_body<PL, SUMS, STATS, GRIDS, false>(a, lds_raw);
+        first_pass_body<PL, SUMS, STATS, GRIDS, false, false>(a, lds_raw);
 }
 
 } // namespace
@@ -1105,7 +1111,7 @@ extern "C" __global__ void __launch_bounds__(256) mcml_refshape_pass(RefShapeArg
             if (!bnd) { /* interact<false>: s > 0 on every uncut hop */
                 n_int++;
                 p.w -= rintf(p.albedo * p.w);
-                spin(p);
+                spin<true>(p);
             }
             if (!bnd) st = survive(p);
             if (st == ST_ALIVE) continue;
