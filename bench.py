@@ -45,11 +45,11 @@ PEAK_FP32_TFLOPS = 157.3
 EXAMPLES = {"head_model": "tests/golden/head_model", "hw8a": "tests/golden/HW8_A"}
 # HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each scaled by what the
 # counter reads for a plain 1 GiB device copy (profiles/transport_cfg1_1e8.md,
-# profiles/surrogate_fused_cfg4.md): transport 4.98e7 B read + 9.09e7 B written per 1e8
+# profiles/surrogate_fused_cfg4.md): transport 5.00e7 B read + 9.10e7 B written per 1e8
 # photons over the pass's six launches (ticket atomics, counters, and the photon states the
 # main launch hands to the compacted drain launches); fused inference 8.55e6 + 1.10e6 B per
 # 1e5 rows.
-TRANSPORT_TRAFFIC_PER_PHOTON = (4.977e7 + 9.089e7) / 1e8
+TRANSPORT_TRAFFIC_PER_PHOTON = (4.997e7 + 9.103e7) / 1e8
 MLP_TRAFFIC_PER_SAMPLE = (8.551e6 + 1.100e6) / 1e5
 HBM_PEAK_GBS = 8000.0
 
