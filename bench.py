@@ -325,7 +325,7 @@ PEAK_BF16_TFLOPS = 2500.0
 
 
 def bench_surrogate(args):
-    """cfg4: train the 14 -> 4x256 GELU -> 3 network on White-MC-expanded batched MC, judge it
+    """cfg4: train the 14 -> 4x256 GELU -> 3 network on White-MC-expanded batched MC, evaluate it
     on held-out sets simulated with plain MC, and time batched inference."""
     import numpy as np
     import mcml_gpu_amd as M
@@ -383,7 +383,7 @@ def bench_surrogate(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
         "data": "synthetic (simulated by this library)",
         "config": {"workload": "cfg4: MLP 14-4x256 GELU-3, inference batch %d; trained on %d samples (%d sets "
-                               "x %.3g photons, White MC x%d); judged on %d held-out sets x %.3g photons"
+                               "x %.3g photons, White MC x%d); evaluated on %d held-out sets x %.3g photons"
                                % (args.infer_batch, len(x), args.train_sets, args.train_photons, args.white,
                                   args.heldout_sets, args.heldout_photons),
                    "epochs": args.epochs},
